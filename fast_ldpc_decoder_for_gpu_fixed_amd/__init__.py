@@ -1,0 +1,325 @@
+"""MI355X-native 8-bit fixed-point Min-Sum LDPC decoder (gfx950).
+
+Python face of ``libldpc_mi355x.so`` (C-ABI: ``include/ldpc_mi355x.h``).  It
+mirrors the reference's decoder objects (``src/decoder_template/CGPUDecoder.h``
+and the five subclasses) so that tests and drivers read like the reference:
+
+    dec = CGPU_Decoder_OMS_SIMD(nb_frames=W, n=N, k=checks, m=edges, code=code)
+    dec.initialize()
+    dec.decode(noise, decoded, 20)
+
+All decoding runs in the HIP library; nothing here computes a decode on the
+CPU, and every entry point raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import LdpcError, Tuning, check, load, ptr
+
+MS, OMS, OMS_V2, NMS, NMS2, LOMS, LNMS = range(7)
+VARIANT_NAMES = {MS: "MS", OMS: "OMS", OMS_V2: "OMS_v2", NMS: "NMS", NMS2: "2NMS",
+                 LOMS: "learned-OMS", LNMS: "learned-NMS"}
+
+
+# ---------------------------------------------------------------------------
+# codes (replaces the compile-time tables of src/matrix)
+# ---------------------------------------------------------------------------
+class Code:
+    """A parity-check table: checks in processing order, each with its variables."""
+
+    def __init__(self, handle: int):
+        if not handle:
+            raise LdpcError(f"code creation failed: {_lib.last_error()}")
+        self._h = C.c_void_p(handle)
+        n, c, e, d, lv, q = (C.c_int() for _ in range(6))
+        check(load().ldpc_code_info(self._h, C.byref(n), C.byref(c), C.byref(e), C.byref(d),
+                                    C.byref(lv), C.byref(q)), "ldpc_code_info")
+        self.n_vars, self.n_checks, self.n_edges = n.value, c.value, e.value
+        self.max_deg, self.n_levels, self.q1_split = d.value, lv.value, q.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.ldpc_code_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n_info(self) -> int:
+        """N - #checks: the positions the reference's error counter scans
+        (src/ber_analyzer/CErrorAnalyzer.cpp:147, _vars = N - K)."""
+        return self.n_vars - self.n_checks
+
+    @property
+    def rate(self) -> float:
+        return self.n_info / self.n_vars
+
+    @classmethod
+    def from_table(cls, n_vars: int, deg, pos, q1_split: int = -1) -> "Code":
+        deg = np.ascontiguousarray(deg, dtype=np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.uint32)
+        if int(deg.sum()) != pos.size:
+            raise LdpcError("sum(deg) != len(pos)")
+        return cls(load().ldpc_code_create(n_vars, deg.size, ptr(deg, C.c_int), ptr(pos, C.c_uint32), q1_split))
+
+    @classmethod
+    def grouped(cls, n_vars, deg1, count1, deg2, count2, pos) -> "Code":
+        pos = np.ascontiguousarray(pos, dtype=np.uint32)
+        return cls(load().ldpc_code_create_grouped(n_vars, deg1, count1, deg2, count2, ptr(pos, C.c_uint32)))
+
+    @classmethod
+    def load_table(cls, path: str, n_vars: int = 0) -> "Code":
+        return cls(load().ldpc_code_load_table(str(path).encode(), n_vars))
+
+    @classmethod
+    def regular(cls, n_vars: int, dv: int, dc: int, seed: int = 1) -> "Code":
+        return cls(load().ldpc_code_regular(n_vars, dv, dc, seed))
+
+    @classmethod
+    def qc(cls, mb: int, nb: int, z: int, p: float, seed: int = 1) -> "Code":
+        return cls(load().ldpc_code_qc(mb, nb, z, p, seed))
+
+    @classmethod
+    def ira(cls, k_info: int, n_hi: int, d_hi: int, d_lo: int, seed: int = 1) -> "Code":
+        return cls(load().ldpc_code_ira(k_info, n_hi, d_hi, d_lo, seed))
+
+    def reorder_by_colour(self) -> "Code":
+        return Code(load().ldpc_code_reorder_by_colour(self._h))
+
+    def export(self):
+        deg = np.empty(self.n_checks, np.int32)
+        pos = np.empty(self.n_edges, np.uint32)
+        check(load().ldpc_code_export(self._h, ptr(deg, C.c_int), ptr(pos, C.c_uint32)), "ldpc_code_export")
+        return deg, pos
+
+
+def code_3_6(n: int = 4000, seed: int = 1, colour: bool = True) -> Code:
+    """(dv=3, dc=6) code from a seeded socket permutation with 4-cycles rejected;
+    by default its table lists checks colour class by colour class."""
+    c = Code.regular(n, 3, 6, seed)
+    return c.reorder_by_colour() if colour else c
+
+
+# ---------------------------------------------------------------------------
+# decoders (replaces class CGPUDecoder and subclasses)
+# ---------------------------------------------------------------------------
+class CGPUDecoder:
+    """src/decoder_template/CGPUDecoder.h:19-35 on the HIP library.
+
+    nb_frames is a count of 32-bit words (4 frames each), as in the reference
+    (src/main.cpp:210); n, k, m are variables, checks, edges and must match
+    the code."""
+
+    VARIANT: Optional[int] = None
+
+    def __init__(self, nb_frames: int, n: int, k: int, m: int, code: Code,
+                 words_per_block: int = 0, threads: int = 0, early_term: int = -1,
+                 post_in_lds: int = -1, variant: Optional[int] = None):
+        if (n, k, m) != (code.n_vars, code.n_checks, code.n_edges):
+            raise LdpcError(f"(n, k, m) = {(n, k, m)} does not match the code "
+                            f"{(code.n_vars, code.n_checks, code.n_edges)}")
+        self.variant = self.VARIANT if variant is None else variant
+        if self.variant is None:
+            raise LdpcError("CGPUDecoder is abstract; use a variant subclass")
+        self.code = code
+        self.nb_frames = nb_frames
+        self.n, self.k, self.m = n, k, m
+        tu = Tuning(words_per_block, threads, early_term, post_in_lds)
+        h = load().ldpc_decoder_create(code.handle, nb_frames, self.variant, C.byref(tu))
+        if not h:
+            raise LdpcError(f"ldpc_decoder_create failed: {_lib.last_error()}")
+        self._h = C.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.ldpc_decoder_destroy(h)
+            self._h = None
+
+    @property
+    def frames(self) -> int:
+        return 4 * self.nb_frames
+
+    @property
+    def io_bytes(self) -> int:
+        return self.frames * self.n
+
+    def initialize(self) -> None:
+        check(load().ldpc_decoder_initialize(self._h), "initialize")
+
+    def _in(self, var_nodes) -> np.ndarray:
+        a = np.ascontiguousarray(var_nodes).view(np.int8).reshape(-1)
+        if a.size != self.io_bytes:
+            raise LdpcError(f"input holds {a.size} bytes, expected {self.io_bytes}")
+        return a
+
+    def decode(self, var_nodes, Rprime_fix, nombre_iterations: int) -> None:
+        """decode(float var_nodes[], int Rprime_fix[], int iterations): the
+        buffers are reinterpreted as bytes, as in the reference."""
+        a = self._in(var_nodes)
+        out = Rprime_fix.view(np.uint8).reshape(-1)
+        if out.size != self.io_bytes or not Rprime_fix.flags.c_contiguous:
+            raise LdpcError("output buffer must be contiguous with 4W*N bytes")
+        check(load().ldpc_decoder_decode(self._h, ptr(a, C.c_int8), ptr(out, C.c_uint8), nombre_iterations), "decode")
+
+    def decode_bytes(self, llr, iterations: int) -> np.ndarray:
+        out = np.empty(self.io_bytes, np.uint8)
+        self.decode(llr, out, iterations)
+        return out
+
+    def decode_device(self, d_in: int, d_out: int, iterations: int, stream: int = 0) -> None:
+        check(load().ldpc_decoder_decode_device(self._h, C.c_void_p(d_in), C.c_void_p(d_out), iterations,
+                                                C.c_void_p(stream) if stream else None), "decode_device")
+
+    @property
+    def stream(self) -> int:
+        return int(load().ldpc_decoder_stream(self._h) or 0)
+
+    def config(self) -> dict:
+        p, t, l, lv = (C.c_int() for _ in range(4))
+        check(load().ldpc_decoder_config(self._h, C.byref(p), C.byref(t), C.byref(l), C.byref(lv)), "config")
+        return {"words_per_block": p.value, "threads": t.value, "post_in_lds": bool(l.value),
+                "n_levels": lv.value}
+
+    def set_table(self, table) -> None:
+        t = np.ascontiguousarray(table, dtype=np.int8)
+        if t.ndim != 2:
+            raise LdpcError("table must be 2-D [iteration][degree]")
+        check(load().ldpc_decoder_set_table(self._h, ptr(t, C.c_int8), t.shape[0], t.shape[1]), "set_table")
+
+    # ---- state API ----
+    def load(self, var_nodes) -> None:
+        a = self._in(var_nodes)
+        check(load().ldpc_decoder_load(self._h, ptr(a, C.c_int8)), "load")
+
+    def iterate(self, iterations: int = 1) -> None:
+        check(load().ldpc_decoder_iterate(self._h, iterations), "iterate")
+
+    def get_state(self):
+        post = np.empty(self.n * self.nb_frames, np.uint32)
+        msg = np.empty(self.m * self.nb_frames, np.uint32)
+        check(load().ldpc_decoder_get_state(self._h, ptr(post, C.c_uint32), ptr(msg, C.c_uint32)), "get_state")
+        return post, msg
+
+    def get_hard(self) -> np.ndarray:
+        out = np.empty(self.io_bytes, np.uint8)
+        check(load().ldpc_decoder_get_hard(self._h, ptr(out, C.c_uint8)), "get_hard")
+        return out
+
+    def stop_info(self):
+        st = np.empty(self.nb_frames, np.uint8)
+        it = np.empty(self.nb_frames, np.int32)
+        check(load().ldpc_decoder_get_stop_info(self._h, ptr(st, C.c_uint8), ptr(it, C.c_int)), "stop_info")
+        return st, it
+
+
+class CGPU_Decoder_MS_SIMD(CGPUDecoder):
+    """src/decoder_ms/CGPU_Decoder_MS_SIMD.cu"""
+    VARIANT = MS
+
+
+class CGPU_Decoder_OMS_SIMD(CGPUDecoder):
+    """src/decoder_oms/CGPU_Decoder_OMS_SIMD.cu"""
+    VARIANT = OMS
+
+
+class CGPU_Decoder_NMS_SIMD(CGPUDecoder):
+    """src/decoder_nms/CGPU_Decoder_NMS_SIMD.cu"""
+    VARIANT = NMS
+
+
+class CGPU_Decoder_2NMS_SIMD(CGPUDecoder):
+    """src/decoder_2nms/CGPU_Decoder_2NMS_SIMD.cu (early stop on by default)"""
+    VARIANT = NMS2
+
+
+class CGPU_Decoder_MS_SIMD_v2(CGPUDecoder):
+    """src/decoder_oms_v2/CGPU_Decoder_MS_SIMD_v2.cu ("xMS"): OMS arithmetic,
+    input consumed in the device layout."""
+    VARIANT = OMS_V2
+
+
+class CGPU_Decoder_Learned_OMS(CGPUDecoder):
+    VARIANT = LOMS
+
+
+class CGPU_Decoder_Learned_NMS(CGPUDecoder):
+    VARIANT = LNMS
+
+
+DECODERS = {MS: CGPU_Decoder_MS_SIMD, OMS: CGPU_Decoder_OMS_SIMD, OMS_V2: CGPU_Decoder_MS_SIMD_v2,
+            NMS: CGPU_Decoder_NMS_SIMD, NMS2: CGPU_Decoder_2NMS_SIMD, LOMS: CGPU_Decoder_Learned_OMS,
+            LNMS: CGPU_Decoder_Learned_NMS}
+
+
+def make_decoder(code: Code, nb_frames: int, variant: int, **tuning) -> CGPUDecoder:
+    return DECODERS[variant](nb_frames, code.n_vars, code.n_checks, code.n_edges, code, **tuning)
+
+
+# ---------------------------------------------------------------------------
+# timer (src/timer/CTimer.cu) and device helpers
+# ---------------------------------------------------------------------------
+class CTimer:
+    def __init__(self, start: bool = False, stream: int = 0):
+        self._t = load().ldpc_timer_create(C.c_void_p(stream) if stream else None)
+        if not self._t:
+            raise LdpcError(_lib.last_error())
+        if start:
+            self.start()
+
+    def start(self):
+        check(load().ldpc_timer_start(self._t), "timer start")
+
+    def stop(self):
+        check(load().ldpc_timer_stop(self._t), "timer stop")
+
+    def get_time_ms(self) -> float:
+        return float(load().ldpc_timer_elapsed_ms(self._t))
+
+    def __del__(self):
+        t = getattr(self, "_t", None)
+        if t and _lib._lib is not None:
+            _lib._lib.ldpc_timer_destroy(t)
+            self._t = None
+
+
+class DeviceBuffer:
+    """Raw device allocation (hipMalloc) owned by Python."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = nbytes
+        self.ptr = load().ldpc_device_alloc(nbytes)
+        if not self.ptr:
+            raise LdpcError(f"device alloc of {nbytes} bytes failed: {_lib.last_error()}")
+
+    def upload(self, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(load().ldpc_memcpy_h2d(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes), "h2d")
+
+    def download(self, arr: np.ndarray) -> np.ndarray:
+        assert arr.flags.c_contiguous and arr.nbytes <= self.nbytes
+        check(load().ldpc_memcpy_d2h(arr.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), arr.nbytes), "d2h")
+        return arr
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p and _lib._lib is not None:
+            _lib._lib.ldpc_device_free(C.c_void_p(p))
+            self.ptr = None
+
+
+def device_count() -> int:
+    return int(load().ldpc_device_count())
+
+
+def synchronize() -> None:
+    check(load().ldpc_device_synchronize(), "synchronize")

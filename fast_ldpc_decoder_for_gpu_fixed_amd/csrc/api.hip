@@ -1,0 +1,615 @@
+// api.hip -- the extern "C" boundary (include/ldpc_mi355x.h) over the HIP
+// kernels: decoder objects, the state API, device layout helpers, timer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ldpc_mi355x.h"
+#include "code.h"
+#include "kernels.h"
+
+using ldpc::Plan;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess)                                                            \
+            return fail(LDPC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+bool device_ok(std::string *why) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        *why = "no HIP device visible";
+        return false;
+    }
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        *why = "cannot query the HIP device";
+        return false;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        *why = std::string("device is ") + prop.gcnArchName + ", library is built for gfx950 only";
+        return false;
+    }
+    return true;
+}
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void **)p, count * sizeof(T));
+}
+
+}  // namespace
+
+struct ldpc_decoder {
+    ldpc_code code;
+    Plan plan;
+    size_t W = 0;
+    int variant = 0;
+    int P = 1, log2P = 0, threads = 256;
+    bool post_in_lds = true;
+    int early_term = 0;
+    hipStream_t stream = nullptr;
+    // device
+    uint2 *d_checks = nullptr;
+    uint32_t *d_edge_var = nullptr;
+    int *d_level_beg = nullptr;
+    uint32_t *d_post = nullptr;
+    uint32_t *d_msgs = nullptr;
+    int8_t *d_in = nullptr;
+    uint8_t *d_out = nullptr;
+    uint8_t *d_stopped = nullptr;
+    int *d_iters = nullptr;
+    int8_t *d_table = nullptr;
+    int tits = 0, tdeg = 0;
+    // state API
+    bool loaded = false;
+    bool pending_first = false;
+    int iters_done = 0;
+};
+
+struct ldpc_timer {
+    hipStream_t stream = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+namespace {
+
+void free_decoder(ldpc_decoder *d) {
+    if (!d) return;
+    hipFree(d->d_checks);
+    hipFree(d->d_edge_var);
+    hipFree(d->d_level_beg);
+    hipFree(d->d_post);
+    hipFree(d->d_msgs);
+    hipFree(d->d_in);
+    hipFree(d->d_out);
+    hipFree(d->d_stopped);
+    hipFree(d->d_iters);
+    hipFree(d->d_table);
+    if (d->stream) hipStreamDestroy(d->stream);
+    delete d;
+}
+
+__global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_i32(int *p, size_t n, int v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+ldpc::DecodeArgs base_args(const ldpc_decoder *d) {
+    ldpc::DecodeArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.checks = d->d_checks;
+    a.edge_var = d->d_edge_var;
+    a.level_beg = d->d_level_beg;
+    a.n_levels = d->plan.n_levels;
+    a.N = d->code.N;
+    a.C = d->code.C;
+    a.E = d->code.E;
+    a.P = d->P;
+    a.log2P = d->log2P;
+    a.W = d->W;
+    a.post_state = d->d_post;
+    a.msgs = d->d_msgs;
+    a.variant = d->variant;
+    a.early_term = d->early_term;
+    a.stopped = d->d_stopped;
+    a.iters_run = d->d_iters;
+    if (d->variant == LDPC_LOMS || d->variant == LDPC_LNMS) {
+        a.ltable = d->d_table;
+        a.tits = d->tits;
+        a.tdeg = d->tdeg;
+    }
+    return a;
+}
+
+int launch(ldpc_decoder *d, const ldpc::DecodeArgs &a, hipStream_t s) {
+    if ((d->variant == LDPC_LOMS || d->variant == LDPC_LNMS) && !d->d_table)
+        return fail(LDPC_ERR_STATE, "learned variant needs ldpc_decoder_set_table first");
+    HIP_TRY(ldpc::launch_decode(a, d->plan.max_deg, d->threads, d->post_in_lds, s));
+    return LDPC_OK;
+}
+
+int reset_counters(ldpc_decoder *d, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u8, dim3(256), dim3(256), 0, s, d->d_stopped, d->W, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, s, d->d_iters, d->W, 0);
+    HIP_TRY(hipGetLastError());
+    return LDPC_OK;
+}
+
+int check_iters(int iterations) {
+    if (iterations < 1) return fail(LDPC_ERR_ARG, "iterations must be >= 1");
+    return LDPC_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char *ldpc_last_error(void) { return g_err.c_str(); }
+const char *ldpc_version(void) { return "ldpc_mi355x 0.1 (gfx950)"; }
+
+// ---- codes ----------------------------------------------------------------
+ldpc_code *ldpc_code_create(int n_vars, int n_checks, const int *deg, const uint32_t *pos, int q1_split) {
+    if (!deg || !pos || n_vars <= 0 || n_checks <= 0) {
+        fail(LDPC_ERR_ARG, "ldpc_code_create: bad arguments");
+        return nullptr;
+    }
+    ldpc_code *c = new ldpc_code();
+    c->N = n_vars;
+    c->C = n_checks;
+    c->deg.assign(deg, deg + n_checks);
+    size_t E = 0;
+    for (int i = 0; i < n_checks; i++) E += deg[i] > 0 ? (size_t)deg[i] : 0;
+    c->pos.assign(pos, pos + E);
+    std::string err = ldpc::finish_code(*c, q1_split);
+    if (!err.empty()) {
+        delete c;
+        fail(LDPC_ERR_ARG, err);
+        return nullptr;
+    }
+    return c;
+}
+
+ldpc_code *ldpc_code_create_grouped(int n_vars, int deg1, int count1, int deg2, int count2, const uint32_t *pos) {
+    if (deg1 < 1 || count1 < 1 || count2 < 0 || (count2 > 0 && deg2 < 1)) {
+        fail(LDPC_ERR_ARG, "ldpc_code_create_grouped: bad degree groups");
+        return nullptr;
+    }
+    std::vector<int> deg((size_t)count1, deg1);
+    deg.insert(deg.end(), (size_t)count2, deg2);
+    return ldpc_code_create(n_vars, count1 + count2, deg.data(), pos, count1);
+}
+
+static ldpc_code *wrap(std::string err, ldpc_code *c) {
+    if (!err.empty()) {
+        delete c;
+        fail(LDPC_ERR_ARG, err);
+        return nullptr;
+    }
+    return c;
+}
+
+ldpc_code *ldpc_code_load_table(const char *path, int n_vars) {
+    if (!path) { fail(LDPC_ERR_ARG, "null path"); return nullptr; }
+    ldpc_code *c = new ldpc_code();
+    return wrap(ldpc::load_table(*c, path, n_vars), c);
+}
+ldpc_code *ldpc_code_regular(int n_vars, int dv, int dc, uint64_t seed) {
+    ldpc_code *c = new ldpc_code();
+    return wrap(ldpc::gen_regular(*c, n_vars, dv, dc, seed), c);
+}
+ldpc_code *ldpc_code_qc(int mb, int nb, int z, double p, uint64_t seed) {
+    ldpc_code *c = new ldpc_code();
+    return wrap(ldpc::gen_qc(*c, mb, nb, z, p, seed), c);
+}
+ldpc_code *ldpc_code_ira(int k_info, int n_hi, int d_hi, int d_lo, uint64_t seed) {
+    ldpc_code *c = new ldpc_code();
+    return wrap(ldpc::gen_ira(*c, k_info, n_hi, d_hi, d_lo, seed), c);
+}
+ldpc_code *ldpc_code_reorder_by_colour(const ldpc_code *code) {
+    if (!code) { fail(LDPC_ERR_ARG, "null code"); return nullptr; }
+    ldpc_code *c = new ldpc_code();
+    return wrap(ldpc::reorder_by_colour(*code, *c), c);
+}
+void ldpc_code_destroy(ldpc_code *code) { delete code; }
+
+int ldpc_code_info(const ldpc_code *code, int *n_vars, int *n_checks, int *n_edges, int *max_deg,
+                   int *n_levels, int *q1_split) {
+    if (!code) return fail(LDPC_ERR_ARG, "null code");
+    if (n_vars) *n_vars = code->N;
+    if (n_checks) *n_checks = code->C;
+    if (n_edges) *n_edges = code->E;
+    if (max_deg) *max_deg = code->max_deg;
+    if (n_levels) *n_levels = ldpc::dependency_levels(*code, nullptr);
+    if (q1_split) *q1_split = code->q1_split;
+    return LDPC_OK;
+}
+
+int ldpc_code_export(const ldpc_code *code, int *deg, uint32_t *pos) {
+    if (!code) return fail(LDPC_ERR_ARG, "null code");
+    if (deg) std::copy(code->deg.begin(), code->deg.end(), deg);
+    if (pos) std::copy(code->pos.begin(), code->pos.end(), pos);
+    return LDPC_OK;
+}
+
+// ---- decoders ---------------------------------------------------------------
+ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words, int variant,
+                                  const ldpc_tuning *tuning) {
+    std::string why;
+    if (!code || nb_frames_words == 0 || variant < LDPC_MS || variant > LDPC_LNMS) {
+        fail(LDPC_ERR_ARG, "ldpc_decoder_create: bad arguments");
+        return nullptr;
+    }
+    if (code->max_deg > ldpc::MAX_DEG) {
+        fail(LDPC_ERR_ARG, "check degree above " + std::to_string(ldpc::MAX_DEG) + " is not supported");
+        return nullptr;
+    }
+    if (!device_ok(&why)) {
+        fail(LDPC_ERR_NODEV, why);
+        return nullptr;
+    }
+    ldpc_tuning tu = {0, 0, -1, -1};
+    if (tuning) tu = *tuning;
+    ldpc_decoder *d = new ldpc_decoder();
+    d->code = *code;
+    d->plan = ldpc::make_plan(d->code);
+    d->W = nb_frames_words;
+    d->variant = variant;
+    d->early_term = tu.early_term >= 0 ? (tu.early_term != 0) : (variant == LDPC_2NMS);
+    const size_t post_word_bytes = (size_t)code->N * 4;
+    const size_t lds_budget = ldpc::MAX_LDS_BYTES - 1024;
+    d->post_in_lds = tu.post_in_lds >= 0 ? (tu.post_in_lds != 0) : (post_word_bytes <= lds_budget);
+    if (d->post_in_lds && post_word_bytes > lds_budget) {
+        free_decoder(d);
+        fail(LDPC_ERR_ARG, "posterior of one word does not fit in LDS");
+        return nullptr;
+    }
+    int P = tu.words_per_block;
+    if (P <= 0) {
+        P = 1;
+        const size_t target = d->post_in_lds ? 80 * 1024 : (size_t)1 << 30;
+        while (P * 2 <= ldpc::MAX_P && post_word_bytes * (size_t)(P * 2) <= target && nb_frames_words % (size_t)(P * 2) == 0)
+            P *= 2;
+    }
+    if (P < 1 || P > ldpc::MAX_P || (P & (P - 1)) || nb_frames_words % (size_t)P ||
+        (d->post_in_lds && post_word_bytes * (size_t)P > lds_budget)) {
+        free_decoder(d);
+        fail(LDPC_ERR_ARG, "words_per_block must be a power of two <= 16 dividing nb_frames and fitting LDS");
+        return nullptr;
+    }
+    d->P = P;
+    while ((1 << d->log2P) < P) d->log2P++;
+    int T = tu.threads > 0 ? tu.threads : 256;
+    T = std::min(T, ldpc::max_threads_for_degree(d->plan.max_deg));
+    if (T % 64 || T % P || T < 64) {
+        free_decoder(d);
+        fail(LDPC_ERR_ARG, "threads must be a multiple of 64 and of words_per_block");
+        return nullptr;
+    }
+    d->threads = T;
+
+    const size_t W = d->W;
+    const ldpc_code &c = d->code;
+    std::vector<uint2> ck((size_t)c.C);
+    for (int i = 0; i < c.C; i++) ck[i] = make_uint2(d->plan.check_info[i], d->plan.check_e0[i]);
+    hipError_t e = hipSuccess;
+#define ALLOC(ptr, n) if (e == hipSuccess) e = dalloc(&d->ptr, (n))
+    ALLOC(d_checks, (size_t)c.C);
+    ALLOC(d_edge_var, (size_t)c.E);
+    ALLOC(d_level_beg, (size_t)d->plan.n_levels + 1);
+    ALLOC(d_post, (size_t)c.N * W);
+    ALLOC(d_msgs, (size_t)c.E * W);
+    ALLOC(d_in, (size_t)c.N * W * 4);
+    ALLOC(d_out, (size_t)c.N * W * 4);
+    ALLOC(d_stopped, W);
+    ALLOC(d_iters, W);
+#undef ALLOC
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemcpy(d->d_checks, ck.data(), ck.size() * sizeof(uint2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->d_edge_var, d->plan.edge_var.data(), (size_t)c.E * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d->d_level_beg, d->plan.level_beg.data(), d->plan.level_beg.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d->d_msgs, 0, (size_t)c.E * W * 4);
+    if (e == hipSuccess) e = hipMemset(d->d_stopped, 0, W);
+    if (e == hipSuccess) e = hipMemset(d->d_iters, 0, W * sizeof(int));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        const bool oom = e == hipErrorOutOfMemory;
+        free_decoder(d);
+        fail(oom ? LDPC_ERR_NOMEM : LDPC_ERR_HIP, std::string("ldpc_decoder_create: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    return d;
+}
+
+int ldpc_decoder_initialize(ldpc_decoder *dec) {
+    if (!dec) return fail(LDPC_ERR_ARG, "null decoder");
+    return LDPC_OK;
+}
+
+void ldpc_decoder_destroy(ldpc_decoder *dec) {
+    if (dec) hipStreamSynchronize(dec->stream);
+    free_decoder(dec);
+}
+
+void *ldpc_decoder_stream(ldpc_decoder *dec) { return dec ? (void *)dec->stream : nullptr; }
+
+int ldpc_decoder_config(const ldpc_decoder *dec, int *words_per_block, int *threads, int *post_in_lds,
+                        int *n_levels) {
+    if (!dec) return fail(LDPC_ERR_ARG, "null decoder");
+    if (words_per_block) *words_per_block = dec->P;
+    if (threads) *threads = dec->threads;
+    if (post_in_lds) *post_in_lds = dec->post_in_lds ? 1 : 0;
+    if (n_levels) *n_levels = dec->plan.n_levels;
+    return LDPC_OK;
+}
+
+int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, int n_degs) {
+    if (!dec || !table || n_iters < 1 || n_degs < 1) return fail(LDPC_ERR_ARG, "bad table");
+    const int lo = 0, hi = dec->variant == LDPC_LNMS ? 8 : 127;
+    for (int i = 0; i < n_iters * n_degs; i++)
+        if (table[i] < lo || table[i] > hi) return fail(LDPC_ERR_ARG, "table value out of range");
+    HIP_TRY(hipStreamSynchronize(dec->stream));
+    hipFree(dec->d_table);
+    dec->d_table = nullptr;
+    HIP_TRY(dalloc(&dec->d_table, (size_t)n_iters * n_degs));
+    HIP_TRY(hipMemcpy(dec->d_table, table, (size_t)n_iters * n_degs, hipMemcpyHostToDevice));
+    dec->tits = n_iters;
+    dec->tdeg = n_degs;
+    return LDPC_OK;
+}
+
+static size_t io_bytes(const ldpc_decoder *d) { return (size_t)d->code.N * d->W * 4; }
+
+int ldpc_decoder_decode_device(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d_out, int iterations, void *stream) {
+    if (!dec || !d_in || !d_out) return fail(LDPC_ERR_ARG, "null argument");
+    int rc = check_iters(iterations);
+    if (rc) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : dec->stream;
+    if (dec->early_term) {
+        rc = reset_counters(dec, s);
+        if (rc) return rc;
+    }
+    ldpc::DecodeArgs a = base_args(dec);
+    if (dec->variant == LDPC_OMS_V2) {
+        a.load_mode = ldpc::LOAD_WORDS;
+        a.in_words = reinterpret_cast<const uint32_t *>(d_in);
+    } else {
+        a.load_mode = ldpc::LOAD_FRAMES;
+        a.in_frames = d_in;
+    }
+    a.out_hard = d_out;
+    a.store_state = 0;
+    a.iters = iterations;
+    a.first = 1;
+    a.iter_base = 0;
+    if (!dec->early_term) { a.stopped = nullptr; a.iters_run = nullptr; }
+    dec->loaded = false;  // the state buffers no longer describe a loaded decode
+    return launch(dec, a, s);
+}
+
+int ldpc_decoder_decode(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations) {
+    if (!dec || !in || !out) return fail(LDPC_ERR_ARG, "null argument");
+    int rc = check_iters(iterations);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(dec->d_in, in, io_bytes(dec), hipMemcpyHostToDevice, dec->stream));
+    rc = ldpc_decoder_decode_device(dec, dec->d_in, dec->d_out, iterations, dec->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, dec->d_out, io_bytes(dec), hipMemcpyDeviceToHost, dec->stream));
+    HIP_TRY(hipStreamSynchronize(dec->stream));
+    return LDPC_OK;
+}
+
+int ldpc_decoder_load(ldpc_decoder *dec, const int8_t *in) {
+    if (!dec || !in) return fail(LDPC_ERR_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(dec->d_in, in, io_bytes(dec), hipMemcpyHostToDevice, dec->stream));
+    int rc = reset_counters(dec, dec->stream);
+    if (rc) return rc;
+    ldpc::DecodeArgs a = base_args(dec);
+    if (dec->variant == LDPC_OMS_V2) {
+        a.load_mode = ldpc::LOAD_WORDS;
+        a.in_words = reinterpret_cast<const uint32_t *>(dec->d_in);
+    } else {
+        a.load_mode = ldpc::LOAD_FRAMES;
+        a.in_frames = dec->d_in;
+    }
+    a.store_state = 1;
+    a.iters = 0;
+    a.first = 1;
+    a.iters_run = nullptr;
+    rc = launch(dec, a, dec->stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(dec->stream));
+    dec->loaded = true;
+    dec->pending_first = true;
+    dec->iters_done = 0;
+    return LDPC_OK;
+}
+
+int ldpc_decoder_iterate(ldpc_decoder *dec, int iterations) {
+    if (!dec) return fail(LDPC_ERR_ARG, "null decoder");
+    int rc = check_iters(iterations);
+    if (rc) return rc;
+    if (!dec->loaded) return fail(LDPC_ERR_STATE, "ldpc_decoder_iterate before ldpc_decoder_load");
+    ldpc::DecodeArgs a = base_args(dec);
+    a.load_mode = ldpc::LOAD_STATE;
+    a.store_state = 1;
+    a.iters = iterations;
+    a.first = dec->pending_first ? 1 : 0;
+    a.iter_base = dec->iters_done;
+    rc = launch(dec, a, dec->stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(dec->stream));
+    dec->pending_first = false;
+    dec->iters_done += iterations;
+    return LDPC_OK;
+}
+
+int ldpc_decoder_get_state(ldpc_decoder *dec, uint32_t *post, uint32_t *msg) {
+    if (!dec) return fail(LDPC_ERR_ARG, "null decoder");
+    if (!dec->loaded) return fail(LDPC_ERR_STATE, "no loaded decode");
+    const ldpc_code &c = dec->code;
+    const size_t W = dec->W, P = (size_t)dec->P, nb = W / P;
+    HIP_TRY(hipStreamSynchronize(dec->stream));
+    if (post) {
+        std::vector<uint32_t> h((size_t)c.N * W);
+        HIP_TRY(hipMemcpy(h.data(), dec->d_post, h.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t b = 0; b < nb; b++)
+            for (int v = 0; v < c.N; v++)
+                for (size_t w = 0; w < P; w++) post[(size_t)v * W + b * P + w] = h[(b * c.N + v) * P + w];
+    }
+    if (msg) {
+        std::vector<uint32_t> h((size_t)c.E * W);
+        HIP_TRY(hipMemcpy(h.data(), dec->d_msgs, h.size() * 4, hipMemcpyDeviceToHost));
+        for (int t = 0; t < c.C; t++) {
+            const int s = dec->plan.slot_of[t];
+            const size_t e0 = dec->plan.check_e0[s];
+            const size_t D = (size_t)c.deg[t];
+            for (size_t j = 0; j < D; j++) {
+                const size_t e = c.eoff[t] + j;
+                for (size_t b = 0; b < nb; b++)
+                    for (size_t w = 0; w < P; w++)
+                        msg[e * W + b * P + w] = h[b * (size_t)c.E * P + e0 * P + w * D + j];
+            }
+        }
+    }
+    return LDPC_OK;
+}
+
+int ldpc_decoder_get_hard(ldpc_decoder *dec, uint8_t *out) {
+    if (!dec || !out) return fail(LDPC_ERR_ARG, "null argument");
+    if (!dec->loaded) return fail(LDPC_ERR_STATE, "no loaded decode");
+    ldpc::DecodeArgs a = base_args(dec);
+    a.load_mode = ldpc::LOAD_STATE;
+    a.store_state = 0;
+    a.iters = 0;
+    a.out_hard = dec->d_out;
+    a.iters_run = nullptr;
+    int rc = launch(dec, a, dec->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, dec->d_out, io_bytes(dec), hipMemcpyDeviceToHost, dec->stream));
+    HIP_TRY(hipStreamSynchronize(dec->stream));
+    return LDPC_OK;
+}
+
+int ldpc_decoder_get_stop_info(ldpc_decoder *dec, uint8_t *stopped, int *iters_run) {
+    if (!dec) return fail(LDPC_ERR_ARG, "null decoder");
+    HIP_TRY(hipStreamSynchronize(dec->stream));
+    if (stopped) HIP_TRY(hipMemcpy(stopped, dec->d_stopped, dec->W, hipMemcpyDeviceToHost));
+    if (iters_run) HIP_TRY(hipMemcpy(iters_run, dec->d_iters, dec->W * sizeof(int), hipMemcpyDeviceToHost));
+    return LDPC_OK;
+}
+
+// ---- device layout helpers -------------------------------------------------
+int ldpc_interleave_device(const int8_t *d_in, uint32_t *d_words, int n_vars, size_t W, void *stream) {
+    if (!d_in || !d_words || n_vars <= 0 || W == 0 || W > 0x7FFFFFFF) return fail(LDPC_ERR_ARG, "bad arguments");
+    HIP_TRY(ldpc::launch_interleave(d_in, d_words, n_vars, (int)W, (hipStream_t)stream));
+    return LDPC_OK;
+}
+int ldpc_deinterleave_hd_device(const uint32_t *d_words, uint8_t *d_out, int n_vars, size_t W, void *stream) {
+    if (!d_words || !d_out || n_vars <= 0 || W == 0 || W > 0x7FFFFFFF) return fail(LDPC_ERR_ARG, "bad arguments");
+    HIP_TRY(ldpc::launch_deinterleave_hd(d_words, d_out, n_vars, (int)W, (hipStream_t)stream));
+    return LDPC_OK;
+}
+
+// ---- memory helpers -----------------------------------------------------------
+void *ldpc_device_alloc(size_t bytes) {
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        fail(e == hipErrorOutOfMemory ? LDPC_ERR_NOMEM : LDPC_ERR_HIP, hipGetErrorString(e));
+        return nullptr;
+    }
+    return p;
+}
+void ldpc_device_free(void *p) { hipFree(p); }
+void *ldpc_host_alloc_pinned(size_t bytes) {
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        fail(LDPC_ERR_HIP, hipGetErrorString(e));
+        return nullptr;
+    }
+    return p;
+}
+void ldpc_host_free_pinned(void *p) { hipHostFree(p); }
+int ldpc_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return LDPC_OK;
+}
+int ldpc_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return LDPC_OK;
+}
+int ldpc_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return LDPC_OK;
+}
+int ldpc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    int good = 0;
+    for (int i = 0; i < n; i++) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, i) == hipSuccess && std::strncmp(p.gcnArchName, "gfx950", 6) == 0) good++;
+    }
+    return good;
+}
+
+// ---- timer ------------------------------------------------------------------------
+ldpc_timer *ldpc_timer_create(void *stream) {
+    ldpc_timer *t = new ldpc_timer();
+    t->stream = (hipStream_t)stream;
+    if (hipEventCreate(&t->a) != hipSuccess || hipEventCreate(&t->b) != hipSuccess) {
+        fail(LDPC_ERR_HIP, "hipEventCreate failed");
+        if (t->a) hipEventDestroy(t->a);
+        delete t;
+        return nullptr;
+    }
+    return t;
+}
+int ldpc_timer_start(ldpc_timer *t) {
+    if (!t) return fail(LDPC_ERR_ARG, "null timer");
+    HIP_TRY(hipEventRecord(t->a, t->stream));
+    return LDPC_OK;
+}
+int ldpc_timer_stop(ldpc_timer *t) {
+    if (!t) return fail(LDPC_ERR_ARG, "null timer");
+    HIP_TRY(hipEventRecord(t->b, t->stream));
+    return LDPC_OK;
+}
+double ldpc_timer_elapsed_ms(ldpc_timer *t) {
+    if (!t) return -1.0;
+    if (hipEventSynchronize(t->b) != hipSuccess) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t->a, t->b) != hipSuccess) return -1.0;
+    return ms;
+}
+void ldpc_timer_destroy(ldpc_timer *t) {
+    if (!t) return;
+    hipEventDestroy(t->a);
+    hipEventDestroy(t->b);
+    delete t;
+}
+
+}  // extern "C"

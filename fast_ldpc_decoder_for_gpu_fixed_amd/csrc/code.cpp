@@ -1,0 +1,353 @@
+// code.cpp -- runtime code tables, seeded generators, colouring and the
+// level planner (host C++).
+#include "code.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+
+namespace ldpc {
+
+namespace {
+
+// splitmix64: small, fast, and identical on every platform.
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    // unbiased integer in [0, n)
+    uint32_t below(uint32_t n) {
+        if (n <= 1) return 0;
+        const uint64_t lim = (~0ull) - ((~0ull) % n);
+        uint64_t x;
+        do x = next(); while (x >= lim);
+        return (uint32_t)(x % n);
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+}  // namespace
+
+std::string finish_code(ldpc_code &c, int q1_split) {
+    if (c.N <= 0 || c.C <= 0) return "code needs at least one variable and one check";
+    if ((int)c.deg.size() != c.C) return "degree list length != number of checks";
+    c.eoff.assign((size_t)c.C + 1, 0);
+    c.max_deg = 0;
+    for (int i = 0; i < c.C; i++) {
+        if (c.deg[i] < 1) return "check degree must be >= 1";
+        c.eoff[i + 1] = c.eoff[i] + (size_t)c.deg[i];
+        c.max_deg = std::max(c.max_deg, c.deg[i]);
+    }
+    if (c.eoff[c.C] != c.pos.size()) return "edge list length != sum of check degrees";
+    if (c.pos.size() > 0x7FFFFFFFull) return "too many edges";
+    c.E = (int)c.pos.size();
+    for (uint32_t v : c.pos)
+        if (v >= (uint32_t)c.N) return "variable index out of range";
+    if (q1_split < 0) {
+        int i = 1;
+        while (i < c.C && c.deg[i] == c.deg[0]) i++;
+        q1_split = i;  // == C for a single-degree code
+    }
+    c.q1_split = std::min(q1_split, c.C);
+    return "";
+}
+
+int dependency_levels(const ldpc_code &c, std::vector<int> *level_of) {
+    std::vector<int> last((size_t)c.N, -1);
+    if (level_of) level_of->assign((size_t)c.C, 0);
+    int n_levels = 0;
+    for (int i = 0; i < c.C; i++) {
+        int lvl = 0;
+        for (size_t e = c.eoff[i]; e < c.eoff[i + 1]; e++) lvl = std::max(lvl, last[c.pos[e]] + 1);
+        for (size_t e = c.eoff[i]; e < c.eoff[i + 1]; e++) last[c.pos[e]] = lvl;
+        if (level_of) (*level_of)[i] = lvl;
+        n_levels = std::max(n_levels, lvl + 1);
+    }
+    return n_levels;
+}
+
+Plan make_plan(const ldpc_code &c) {
+    Plan p;
+    std::vector<int> lvl;
+    p.n_levels = dependency_levels(c, &lvl);
+    p.max_deg = c.max_deg;
+    p.order.resize((size_t)c.C);
+    std::iota(p.order.begin(), p.order.end(), 0);
+    // by level, then degree descending (keeps a wavefront on one degree), then table index
+    std::stable_sort(p.order.begin(), p.order.end(), [&](int a, int b) {
+        if (lvl[a] != lvl[b]) return lvl[a] < lvl[b];
+        return c.deg[a] > c.deg[b];
+    });
+    p.level_beg.assign((size_t)p.n_levels + 1, 0);
+    for (int i = 0; i < c.C; i++) p.level_beg[lvl[i] + 1]++;
+    for (int l = 0; l < p.n_levels; l++) p.level_beg[l + 1] += p.level_beg[l];
+    p.slot_of.resize((size_t)c.C);
+    p.check_info.resize((size_t)c.C);
+    p.check_e0.resize((size_t)c.C);
+    p.edge_var.resize((size_t)c.E);
+    uint32_t e0 = 0;
+    for (int s = 0; s < c.C; s++) {
+        const int t = p.order[s];
+        p.slot_of[t] = s;
+        p.check_info[s] = (uint32_t)c.deg[t] | ((t >= c.q1_split) ? 0x100u : 0u);
+        p.check_e0[s] = e0;
+        for (int j = 0; j < c.deg[t]; j++) p.edge_var[e0 + j] = c.pos[c.eoff[t] + j];
+        e0 += (uint32_t)c.deg[t];
+    }
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// regular (dv, dc) code: check sockets filled one by one from the pool of
+// variable sockets (a random permutation drawn lazily), rejecting a variable
+// already in the check or one that would close a 4-cycle.
+// ---------------------------------------------------------------------------
+std::string gen_regular(ldpc_code &out, int n, int dv, int dc, uint64_t seed) {
+    if (n <= 0 || dv <= 0 || dc <= 1) return "bad regular code parameters";
+    if (((long long)n * dv) % dc) return "n*dv must be a multiple of dc";
+    const int C = (int)((long long)n * dv / dc);
+    for (int attempt = 0; attempt < 64; attempt++) {
+        Rng rng(seed * 0x9E3779B97F4A7C15ull + (uint64_t)attempt * 0xD1B54A32D192ED03ull + 1);
+        std::vector<uint32_t> pool((size_t)n * dv);
+        for (int v = 0; v < n; v++)
+            for (int k = 0; k < dv; k++) pool[(size_t)v * dv + k] = (uint32_t)v;
+        size_t remaining = pool.size();
+        std::vector<std::vector<int>> var_checks((size_t)n);
+        std::vector<uint32_t> pos;
+        pos.reserve(pool.size());
+        bool ok = true;
+        std::vector<char> mark((size_t)n, 0);  // variables sharing a check with the current check's vars
+        for (int c = 0; c < C && ok; c++) {
+            const size_t base = pos.size();
+            for (int j = 0; j < dc && ok; j++) {
+                // variables forbidden for this slot: in the check, or 2 hops away
+                auto allowed = [&](uint32_t v) {
+                    for (size_t e = base; e < pos.size(); e++)
+                        if (pos[e] == v) return false;
+                    return !mark[v];
+                };
+                size_t pick = remaining;
+                for (int tries = 0; tries < 64 && pick == remaining; tries++) {
+                    const size_t r = rng.below((uint32_t)remaining);
+                    if (allowed(pool[r])) pick = r;
+                }
+                if (pick == remaining) {
+                    const size_t start = rng.below((uint32_t)remaining);
+                    for (size_t k = 0; k < remaining; k++) {
+                        const size_t r = (start + k) % remaining;
+                        if (allowed(pool[r])) { pick = r; break; }
+                    }
+                }
+                if (pick == remaining) { ok = false; break; }
+                const uint32_t v = pool[pick];
+                pool[pick] = pool[remaining - 1];
+                remaining--;
+                pos.push_back(v);
+                // every variable sharing an earlier check with v is now 2 hops away
+                for (int oc : var_checks[v]) {
+                    const size_t b0 = (size_t)oc * dc;
+                    for (int k = 0; k < dc; k++) mark[pos[b0 + k]] = 1;
+                }
+            }
+            if (!ok) break;
+            for (int j = 0; j < dc; j++) var_checks[pos[base + j]].push_back(c);
+            // clear marks of this check's neighbourhood
+            for (int j = 0; j < dc; j++)
+                for (int oc : var_checks[pos[base + j]]) {
+                    const size_t b0 = (size_t)oc * dc;
+                    for (int k = 0; k < dc && b0 + k < pos.size(); k++) mark[pos[b0 + k]] = 0;
+                }
+            std::sort(pos.begin() + (long)base, pos.end());
+        }
+        if (!ok) continue;
+        out = ldpc_code();
+        out.N = n;
+        out.C = C;
+        out.deg.assign((size_t)C, dc);
+        out.pos = std::move(pos);
+        return finish_code(out, -1);
+    }
+    return "could not build a 4-cycle-free regular code (64 attempts)";
+}
+
+// ---------------------------------------------------------------------------
+// quasi-cyclic code from a random base graph with a dual-diagonal parity part
+// ---------------------------------------------------------------------------
+std::string gen_qc(ldpc_code &out, int mb, int nb, int z, double p, uint64_t seed) {
+    if (mb < 2 || nb <= mb || z < 1 || p < 0 || p > 1) return "bad QC parameters";
+    Rng rng(seed ^ 0x51C0DE5EEDull);
+    const int kb = nb - mb;
+    std::vector<int> base((size_t)mb * nb, -1);
+    for (int c = 0; c < kb; c++) {
+        int cnt = 0;
+        for (int attempt = 0; attempt < 1000 && cnt == 0; attempt++) {
+            for (int r = 0; r < mb; r++) {
+                const bool present = rng.uniform() < p;
+                const int shift = (int)rng.below((uint32_t)z);
+                base[(size_t)r * nb + c] = present ? shift : -1;
+                cnt += present;
+            }
+        }
+        if (cnt == 0) base[(size_t)rng.below((uint32_t)mb) * nb + c] = (int)rng.below((uint32_t)z);
+    }
+    // parity part: first parity column has shift 1 at the top and bottom and
+    // 0 in the middle; the others form a dual diagonal of identities.
+    base[(size_t)0 * nb + kb] = 1 % z;
+    base[(size_t)(mb / 2) * nb + kb] = 0;
+    base[(size_t)(mb - 1) * nb + kb] = 1 % z;
+    for (int i = 0; i + 1 < mb; i++) {
+        base[(size_t)i * nb + kb + 1 + i] = 0;
+        base[(size_t)(i + 1) * nb + kb + 1 + i] = 0;
+    }
+    out = ldpc_code();
+    out.N = nb * z;
+    out.C = mb * z;
+    for (int r = 0; r < mb; r++)
+        for (int i = 0; i < z; i++) {
+            int d = 0;
+            for (int c = 0; c < nb; c++) {
+                const int s = base[(size_t)r * nb + c];
+                if (s < 0) continue;
+                out.pos.push_back((uint32_t)(c * z + (i + s) % z));
+                d++;
+            }
+            out.deg.push_back(d);
+        }
+    return finish_code(out, -1);
+}
+
+// ---------------------------------------------------------------------------
+// IRA staircase code
+// ---------------------------------------------------------------------------
+std::string gen_ira(ldpc_code &out, int k, int n_hi, int d_hi, int d_lo, uint64_t seed) {
+    if (k < 2 || n_hi < 0 || n_hi > k || d_hi < 1 || d_lo < 1) return "bad IRA parameters";
+    const long long info_edges = (long long)n_hi * d_hi + (long long)(k - n_hi) * d_lo;
+    if (info_edges % k) return "info edges must be a multiple of the number of checks";
+    const int dci = (int)(info_edges / k);
+    if (dci < 1) return "check info degree must be >= 1";
+    std::vector<uint32_t> sock;
+    sock.reserve((size_t)info_edges);
+    for (int v = 0; v < k; v++) {
+        const int d = v < n_hi ? d_hi : d_lo;
+        for (int j = 0; j < d; j++) sock.push_back((uint32_t)v);
+    }
+    Rng rng(seed ^ 0x1EA5EEDull);
+    for (size_t i = sock.size(); i > 1; i--) std::swap(sock[i - 1], sock[rng.below((uint32_t)i)]);
+    // repair repeated variables inside a check by swapping with a random slot
+    for (int pass = 0; pass < 100; pass++) {
+        bool clean = true;
+        for (int c = 0; c < k; c++) {
+            uint32_t *s = &sock[(size_t)c * dci];
+            for (int a = 0; a < dci; a++)
+                for (int b = a + 1; b < dci; b++)
+                    if (s[a] == s[b]) {
+                        clean = false;
+                        std::swap(s[b], sock[rng.below((uint32_t)sock.size())]);
+                    }
+        }
+        if (clean) break;
+        if (pass == 99) return "could not remove repeated variables";
+    }
+    out = ldpc_code();
+    out.N = 2 * k;
+    out.C = k;
+    for (int c = 0; c < k; c++) {
+        std::vector<uint32_t> vs(sock.begin() + (long)c * dci, sock.begin() + (long)(c + 1) * dci);
+        std::sort(vs.begin(), vs.end());
+        if (c > 0) vs.push_back((uint32_t)(k + c - 1));
+        vs.push_back((uint32_t)(k + c));
+        out.deg.push_back((int)vs.size());
+        out.pos.insert(out.pos.end(), vs.begin(), vs.end());
+    }
+    return finish_code(out, -1);
+}
+
+// ---------------------------------------------------------------------------
+// greedy colouring of the conflict graph; table rewritten colour by colour
+// ---------------------------------------------------------------------------
+std::string reorder_by_colour(const ldpc_code &in, ldpc_code &out) {
+    std::vector<std::vector<int>> var_checks((size_t)in.N);
+    for (int c = 0; c < in.C; c++)
+        for (size_t e = in.eoff[c]; e < in.eoff[c + 1]; e++) var_checks[in.pos[e]].push_back(c);
+    std::vector<int> colour((size_t)in.C, -1);
+    std::vector<int> seen;  // seen[col] == c+1 when colour col is taken by a neighbour of c
+    int n_col = 0;
+    for (int c = 0; c < in.C; c++) {
+        for (size_t e = in.eoff[c]; e < in.eoff[c + 1]; e++)
+            for (int o : var_checks[in.pos[e]]) {
+                const int col = colour[o];
+                if (col >= 0) {
+                    if ((int)seen.size() <= col) seen.resize((size_t)col + 1, 0);
+                    seen[col] = c + 1;
+                }
+            }
+        int col = 0;
+        while (col < (int)seen.size() && seen[col] == c + 1) col++;
+        colour[c] = col;
+        n_col = std::max(n_col, col + 1);
+    }
+    std::vector<int> order((size_t)in.C);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return colour[a] < colour[b]; });
+    out = ldpc_code();
+    out.N = in.N;
+    out.C = in.C;
+    for (int c : order) {
+        out.deg.push_back(in.deg[c]);
+        out.pos.insert(out.pos.end(), in.pos.begin() + (long)in.eoff[c], in.pos.begin() + (long)in.eoff[c + 1]);
+    }
+    return finish_code(out, -1);
+}
+
+// ---------------------------------------------------------------------------
+// reference table text: "/* msg = i, deg = d */  v0, v1, ..., " repeated
+// (src/matrix/4000x2000/constantes_decoder.h:3-)
+// ---------------------------------------------------------------------------
+std::string load_table(ldpc_code &out, const char *path, int n_vars) {
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return std::string("cannot open ") + path;
+    std::string text;
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, got);
+    std::fclose(f);
+    out = ldpc_code();
+    size_t i = 0;
+    const size_t n = text.size();
+    uint32_t vmax = 0;
+    while (true) {
+        const size_t k = text.find("deg", i);
+        if (k == std::string::npos) break;
+        size_t q = text.find('=', k);
+        if (q == std::string::npos) break;
+        const int d = std::atoi(text.c_str() + q + 1);
+        size_t end = text.find("*/", q);
+        if (end == std::string::npos || d < 1) return "malformed table near offset " + std::to_string(k);
+        i = end + 2;
+        for (int j = 0; j < d; j++) {
+            while (i < n && !(text[i] >= '0' && text[i] <= '9')) {
+                if (text[i] == '/' && i + 1 < n && text[i + 1] == '*') return "short check in table";
+                i++;
+            }
+            if (i >= n) return "table ends inside a check";
+            const uint32_t v = (uint32_t)std::strtoul(text.c_str() + i, nullptr, 10);
+            while (i < n && text[i] >= '0' && text[i] <= '9') i++;
+            out.pos.push_back(v);
+            vmax = std::max(vmax, v);
+        }
+        out.deg.push_back(d);
+    }
+    if (out.deg.empty()) return "no checks found in table";
+    out.C = (int)out.deg.size();
+    out.N = n_vars > 0 ? n_vars : (int)vmax + 1;
+    return finish_code(out, -1);
+}
+
+}  // namespace ldpc

@@ -1,0 +1,458 @@
+// decode_kernels.hip -- layered 8-bit Min-Sum decoding on gfx950.
+//
+// What is computed (per 32-bit word = 4 frames, per check, in the serial
+// table order of the reference; src/decoder_oms/cuda/CUDA_OMS_SIMD.cu:148-191):
+//     t_j  = sat8(post[v_j] - msg_j)            (first iteration: post[v_j])
+//     min1, min2 = two smallest |t_j| (unsigned, |-128| = 128, both start at 127)
+//     S    = parity of {t_j > 0}
+//     msg_j = +-(|t_j| == min1 ? f(min2) : g(min1)),  sign = parity of the others
+//     post[v_j] = sat8(t_j + msg_j)
+//
+// How it is laid out on the chip:
+//   * one workgroup owns a block of P words (4P frames) for the whole decode;
+//     its posterior lives in LDS as [var][P] words;
+//   * the table is cut into levels (no two checks of a level share a
+//     variable, and every check comes after all earlier-table checks it
+//     shares a variable with), so a level's checks run in parallel and the
+//     result is bit-identical to the serial order; a workgroup barrier
+//     separates levels;
+//   * lane (check, word) reads its D messages as one contiguous run
+//     (messages are stored [block][level-order edge][word][j]) and gathers /
+//     scatters its D posterior words in LDS;
+//   * int8 lanes are computed two frames per packed-16 op (byte_lanes.h).
+#include "byte_lanes.h"
+#include "kernels.h"
+
+namespace ldpc {
+
+// ---------------------------------------------------------------------------
+// per-check magnitudes  (cste_1 from min2, cste_2 from min1)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void magnitudes(int variant, u16x2 min1, u16x2 min2, bool nocap,
+                                           int lparam, u16x2 &k1, u16x2 &k2) {
+    const u16x2 cap31 = splat(0x1F00u);
+    switch (variant) {
+    case VAR_MS:  // CUDA_MS_SIMD.cu:173-174
+        k1 = umin(min2, cap31);
+        k2 = umin(min1, cap31);
+        break;
+    case VAR_OMS:      // CUDA_OMS_SIMD.cu:173-174; no cap: :115-116
+    case VAR_OMS_V2: {  // CUDA_OMS_SIMD_v2.cu:145-146
+        const u16x2 cap = nocap ? splat(0x7F00u) : cap31;
+        const u16x2 one = splat(0x0100u);
+        k1 = umin(sub_sat_u(min2, one), cap);
+        k2 = umin(sub_sat_u(min1, one), cap);
+        break;
+    }
+    case VAR_LOMS: {  // learned offset beta (LSB), cap 31
+        const u16x2 off = splat((uint32_t)lparam << 8);
+        k1 = umin(sub_sat_u(min2, off), cap31);
+        k2 = umin(sub_sat_u(min1, off), cap31);
+        break;
+    }
+    case VAR_NMS: {  // (char)(0.75f*m) = floor(3m/4): m<<8 - m<<6 keeps 3m/4 exactly
+        const u16x2 hi = splat(0xFF00u);
+        k1 = (min2 - (min2 >> (u16x2)(2))) & hi;
+        k2 = (min1 - (min1 >> (u16x2)(2))) & hi;
+        break;
+    }
+    case VAR_2NMS: {  // 0.875 on min2, 0.75 on min1 (CUDA_2NMS_SIMD.cu:73-85)
+        const u16x2 hi = splat(0xFF00u);
+        k1 = (min2 - (min2 >> (u16x2)(3))) & hi;
+        k2 = (min1 - (min1 >> (u16x2)(2))) & hi;
+        break;
+    }
+    default: {  // VAR_LNMS: floor(m * alpha / 8), alpha in [0, 8]
+        const u16x2 hi = splat(0xFF00u);
+        const u16x2 a32 = splat((uint32_t)lparam << 5);
+        k1 = ((min2 >> (u16x2)(8)) * a32) & hi;
+        k2 = ((min1 >> (u16x2)(8)) * a32) & hi;
+        break;
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one check of degree D on one word.
+// EQ = false: r = k1 - min(a - min1, k1 - k2).  Valid when k1 - k2 <= min2 - min1
+//   (every 1-Lipschitz magnitude map: MS, OMS, NMS, learned tables): a lane
+//   that is not the minimum has a >= min2.  The residue 0x7FFF (a saturated
+//   +127, or |-128|) only occurs with a >= 127 >= min2, so it lands on k2 as
+//   the reference's lane does, or on k1 == k2 when min1 == min2 == 127.
+// EQ = true (2NMS, k1 - k2 can exceed min2 - min1): t is cleaned to its high
+//   byte and the test a == min1 is made explicit.
+// Returns the per-frame parity-of-positives bits (0x80008000 positions).
+// ---------------------------------------------------------------------------
+template <int D, bool EQ, typename PostPtr>
+__device__ __forceinline__ uint32_t check_d(PostPtr post, const uint32_t *__restrict__ ev,
+                                            uint32_t *__restrict__ mp, int P, int w, bool first,
+                                            int variant, bool nocap, int lparam) {
+    constexpr bool KEEP = D <= 12;  // keep |t| and -t live across the two passes
+    uint32_t vi[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) vi[j] = ev[j] * (uint32_t)P + (uint32_t)w;
+    uint32_t pw[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) pw[j] = post[vi[j]];
+    uint32_t mw[D];
+    if (!first) {
+#pragma unroll
+        for (int j = 0; j < D; j++) mw[j] = mp[j];
+    }
+
+    u16x2 te[D], to[D], ne[KEEP ? D : 1], no[KEEP ? D : 1], ae[KEEP ? D : 1], ao[KEEP ? D : 1];
+    u16x2 m1e = splat(0x7F00u), m2e = m1e, m1o = m1e, m2o = m1e;
+    u16x2 se = splat(0u), so = se;
+    const u16x2 zero = splat(0u);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        u16x2 xe = unpack_even(pw[j]), xo = unpack_odd(pw[j]);
+        if (!first) {
+            xe = sub_sat_s(xe, unpack_even(mw[j]));
+            xo = sub_sat_s(xo, unpack_odd(mw[j]));
+        }
+        if (EQ) {
+            xe &= splat(0xFF00u);
+            xo &= splat(0xFF00u);
+        }
+        te[j] = xe;
+        to[j] = xo;
+        const u16x2 ye = sub_sat_s(zero, xe), yo = sub_sat_s(zero, xo);  // -t, top bit = (t > 0)
+        const u16x2 be = umin(xe, ye), bo = umin(xo, yo);              // |t| as uint
+        se ^= ye;
+        so ^= yo;
+        m2e = umin(m2e, umax(be, m1e));
+        m1e = umin(m1e, be);
+        m2o = umin(m2o, umax(bo, m1o));
+        m1o = umin(m1o, bo);
+        if constexpr (KEEP) {
+            ne[j] = ye; no[j] = yo; ae[j] = be; ao[j] = bo;
+        }
+    }
+    const uint32_t parity = (w32(se) | w32(so)) & 0x80008000u;
+
+    u16x2 k1e, k2e, k1o, k2o;
+    magnitudes(variant, m1e, m2e, nocap, lparam, k1e, k2e);
+    magnitudes(variant, m1o, m2o, nocap, lparam, k1o, k2o);
+    const u16x2 Ke = k1e - k2e, Ko = k1o - k2o;
+    se ^= splat(0x8000u);  // top bit now: parity of the others is even -> negate
+    so ^= splat(0x8000u);
+
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        u16x2 ye, yo, be, bo;
+        if constexpr (KEEP) {
+            ye = ne[j]; yo = no[j]; be = ae[j]; bo = ao[j];
+        } else {
+            ye = sub_sat_s(zero, te[j]); yo = sub_sat_s(zero, to[j]);
+            be = umin(te[j], ye); bo = umin(to[j], yo);
+        }
+        const u16x2 de = be - m1e, dd = bo - m1o;  // >= 0
+        u16x2 re, ro;
+        if (EQ) {
+            re = k2e + sub_sat_u(splat(0x0001u), de) * Ke;
+            ro = k2o + sub_sat_u(splat(0x0001u), dd) * Ko;
+        } else {
+            re = k1e - umin(de, Ke);
+            ro = k1o - umin(dd, Ko);
+        }
+        const u16x2 se_j = sign_mask(se ^ ye), so_j = sign_mask(so ^ yo);
+        const u16x2 me = (re ^ se_j) - se_j, mo = (ro ^ so_j) - so_j;
+        mp[j] = repack(me, mo);
+        post[vi[j]] = repack(add_sat_s(te[j], me), add_sat_s(to[j], mo));
+    }
+    return parity;
+}
+
+template <bool EQ, int DMAXC, typename PostPtr>
+__device__ __forceinline__ uint32_t check_any(int D, PostPtr post, const uint32_t *__restrict__ ev,
+                                              uint32_t *__restrict__ mp, int P, int w, bool first,
+                                              int variant, bool nocap, int lparam) {
+    switch (D) {
+#define LDPC_CASE(d)                                                                        \
+    case d:                                                                                 \
+        if constexpr (d <= DMAXC)                                                           \
+            return check_d<d, EQ>(post, ev, mp, P, w, first, variant, nocap, lparam);      \
+        else                                                                                \
+            return 0;
+        LDPC_CASE(1) LDPC_CASE(2) LDPC_CASE(3) LDPC_CASE(4) LDPC_CASE(5) LDPC_CASE(6)
+        LDPC_CASE(7) LDPC_CASE(8) LDPC_CASE(9) LDPC_CASE(10) LDPC_CASE(11) LDPC_CASE(12)
+        LDPC_CASE(13) LDPC_CASE(14) LDPC_CASE(15) LDPC_CASE(16) LDPC_CASE(17) LDPC_CASE(18)
+        LDPC_CASE(19) LDPC_CASE(20) LDPC_CASE(21) LDPC_CASE(22) LDPC_CASE(23) LDPC_CASE(24)
+        LDPC_CASE(25) LDPC_CASE(26) LDPC_CASE(27) LDPC_CASE(28) LDPC_CASE(29) LDPC_CASE(30)
+        LDPC_CASE(31) LDPC_CASE(32)
+#undef LDPC_CASE
+    default: return 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// posterior block load / store (LDS [var][P] <-> the three global layouts)
+// ---------------------------------------------------------------------------
+__device__ void load_block(const DecodeArgs &a, uint32_t *post, size_t b) {
+    const int T = blockDim.x, tid = threadIdx.x, P = a.P, N = a.N;
+    if (a.load_mode == LOAD_FRAMES) {
+        // frames [4bP, 4bP+4P) are contiguous: 4P*N bytes
+        const int8_t *src = a.in_frames + b * (size_t)(4 * P) * (size_t)N;
+        uint8_t *lb = reinterpret_cast<uint8_t *>(post);
+        if ((N & 3) == 0) {
+            const int n4 = N >> 2;
+            for (int fl = 0; fl < 4 * P; fl++) {
+                const uint32_t *s32 = reinterpret_cast<const uint32_t *>(src + (size_t)fl * N);
+                const int wl = fl >> 2, k = fl & 3;
+                for (int q = tid; q < n4; q += T) {
+                    const uint32_t x = s32[q];
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        lb[((size_t)(4 * q + r) * P + wl) * 4 + k] = (uint8_t)(x >> (8 * r));
+                }
+            }
+        } else {
+            for (int fl = 0; fl < 4 * P; fl++) {
+                const int wl = fl >> 2, k = fl & 3;
+                for (int v = tid; v < N; v += T)
+                    lb[((size_t)v * P + wl) * 4 + k] = (uint8_t)src[(size_t)fl * N + v];
+            }
+        }
+    } else if (a.load_mode == LOAD_WORDS) {
+        const int lp = a.log2P;
+        for (int i = tid; i < N * P; i += T) {
+            const int v = i >> lp, wl = i & (P - 1);
+            post[i] = a.in_words[(size_t)v * a.W + b * P + wl];
+        }
+    } else {
+        const uint32_t *src = a.post_state + b * (size_t)N * P;
+        for (int i = tid; i < N * P; i += T) post[i] = src[i];
+    }
+}
+
+__device__ void store_block(const DecodeArgs &a, const uint32_t *post, size_t b) {
+    const int T = blockDim.x, tid = threadIdx.x, P = a.P, N = a.N;
+    if (a.store_state) {
+        uint32_t *dst = a.post_state + b * (size_t)N * P;
+        for (int i = tid; i < N * P; i += T) dst[i] = post[i];
+    }
+    if (a.out_hard) {
+        uint8_t *dst = a.out_hard + b * (size_t)(4 * P) * (size_t)N;
+        const uint8_t *lb = reinterpret_cast<const uint8_t *>(post);
+        if ((N & 3) == 0) {
+            const int n4 = N >> 2;
+            for (int fl = 0; fl < 4 * P; fl++) {
+                uint32_t *d32 = reinterpret_cast<uint32_t *>(dst + (size_t)fl * N);
+                const int wl = fl >> 2, k = fl & 3;
+                for (int q = tid; q < n4; q += T) {
+                    uint32_t x = 0;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int8_t s = (int8_t)lb[((size_t)(4 * q + r) * P + wl) * 4 + k];
+                        x |= (uint32_t)(s > 0) << (8 * r);  // GPU_Transpose_uint8.cu:29
+                    }
+                    d32[q] = x;
+                }
+            }
+        } else {
+            for (int fl = 0; fl < 4 * P; fl++) {
+                const int wl = fl >> 2, k = fl & 3;
+                for (int v = tid; v < N; v += T) {
+                    const int8_t s = (int8_t)lb[((size_t)v * P + wl) * 4 + k];
+                    dst[(size_t)fl * N + v] = (uint8_t)(s > 0);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the decode kernel: one workgroup per block of P words, all iterations
+// ---------------------------------------------------------------------------
+template <bool EQ, bool POST_IN_LDS, int DMAXC>
+__global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_decode(const DecodeArgs a) {
+    extern __shared__ uint32_t lds_post[];
+    __shared__ uint32_t s_ov[MAX_P];
+    __shared__ int s_active[MAX_P];
+    __shared__ int8_t s_lrow[MAX_DEG + 1];
+
+    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P;
+    const size_t b = blockIdx.x;
+    const int w = tid & (P - 1);
+    uint32_t *post = POST_IN_LDS ? lds_post : a.post_state + b * (size_t)a.N * P;
+
+    if (POST_IN_LDS || a.load_mode != LOAD_STATE) {
+        if (POST_IN_LDS) {
+            load_block(a, post, b);
+        } else {
+            // global-posterior path: bring the input into the state block
+            const int N = a.N;
+            if (a.load_mode == LOAD_WORDS) {
+                for (int i = tid; i < N * P; i += T) {
+                    const int v = i >> lp, wl = i & (P - 1);
+                    post[i] = a.in_words[(size_t)v * a.W + b * P + wl];
+                }
+            } else {
+                const int8_t *src = a.in_frames + b * (size_t)(4 * P) * (size_t)N;
+                uint8_t *pb = reinterpret_cast<uint8_t *>(post);
+                for (int fl = 0; fl < 4 * P; fl++) {
+                    const int wl = fl >> 2, k = fl & 3;
+                    for (int v = tid; v < N; v += T)
+                        pb[((size_t)v * P + wl) * 4 + k] = (uint8_t)src[(size_t)fl * N + v];
+                }
+            }
+        }
+    }
+    if (tid < P) s_active[tid] = a.stopped ? (a.stopped[b * P + tid] == 0) : 1;
+    __syncthreads();
+
+    uint32_t *mbase = a.msgs + b * (size_t)a.E * P;
+    const int cstride = T >> lp;
+    int ran = 0;
+    for (int it = 0; it < a.iters; it++) {
+        const bool first = a.first && it == 0;
+        if (a.ltable) {
+            const int row = min(a.iter_base + it, a.tits - 1);
+            for (int d = tid; d <= MAX_DEG; d += T)
+                s_lrow[d] = a.ltable[(size_t)row * a.tdeg + min(d, a.tdeg - 1)];
+        }
+        if (tid < P) s_ov[tid] = 0;
+        __syncthreads();
+        const bool act = s_active[w] != 0;
+        uint32_t ov = 0;
+        for (int L = 0; L < a.n_levels; L++) {
+            const int lb = a.level_beg[L], le = a.level_beg[L + 1];
+            if (act) {
+                for (int c = lb + (tid >> lp); c < le; c += cstride) {
+                    const uint2 ci = a.checks[c];  // x: degree | q1 << 8, y: first edge
+                    const int D = (int)(ci.x & 0xFFu);
+                    const bool nocap = first && (ci.x & 0x100u) && a.variant == VAR_OMS;
+                    const int lparam = a.ltable ? (int)s_lrow[D] : 0;
+                    ov |= check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y,
+                                        mbase + (size_t)ci.y * P + (size_t)w * D, P, w, first,
+                                        a.variant, nocap, lparam);
+                }
+            }
+            __syncthreads();
+        }
+        if (act) ran++;
+        if (a.early_term && !first) {
+            if (act && ov) atomicOr(&s_ov[w], 1u);
+            __syncthreads();
+            if (tid < P && s_active[tid] && s_ov[tid] == 0) {
+                s_active[tid] = 0;
+                if (a.stopped) a.stopped[b * P + tid] = 1;
+            }
+            __syncthreads();
+            int any = 0;
+            for (int q = 0; q < P; q++) any |= s_active[q];
+            if (!any) break;
+        }
+    }
+    if (a.iters_run && tid < P) a.iters_run[b * P + tid] += ran;  // tid < P => w == tid
+
+    if (POST_IN_LDS) {
+        store_block(a, post, b);
+    } else if (a.out_hard) {
+        __syncthreads();
+        DecodeArgs c = a;
+        c.store_state = 0;
+        store_block(c, post, b);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device interleave (src/transpose/GPU_Transpose_uint8.cu:80-130) and inverse
+// with hard decision (:9-78), as LDS-tiled transposes: a workgroup moves a
+// tile of TV variables x TW words; global reads and writes are contiguous.
+// ---------------------------------------------------------------------------
+constexpr int TV = 64, TW = 64;
+
+__global__ __launch_bounds__(256) void k_interleave(const int8_t *__restrict__ in,
+                                                    uint32_t *__restrict__ out, int N, int W) {
+    __shared__ uint32_t tile[TV][TW + 1];
+    const int v0 = blockIdx.x * TV, w0 = blockIdx.y * TW, tid = threadIdx.x;
+    uint8_t *tb = reinterpret_cast<uint8_t *>(&tile[0][0]);
+    const int rowb = (TW + 1) * 4;
+    // read 4*TW frames x TV bytes
+    for (int i = tid; i < 4 * TW * TV; i += 256) {
+        const int fl = i / TV, vl = i % TV;
+        const int v = v0 + vl, wl = fl >> 2;
+        if (v < N && w0 + wl < W)
+            tb[vl * rowb + fl] = (uint8_t)in[((size_t)(w0 * 4 + fl)) * N + v];
+    }
+    __syncthreads();
+    for (int i = tid; i < TV * TW; i += 256) {
+        const int vl = i / TW, wl = i % TW;
+        if (v0 + vl < N && w0 + wl < W) out[(size_t)(v0 + vl) * W + w0 + wl] = tile[vl][wl];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_deinterleave_hd(const uint32_t *__restrict__ in,
+                                                         uint8_t *__restrict__ out, int N, int W) {
+    __shared__ uint32_t tile[TV][TW + 1];
+    const int v0 = blockIdx.x * TV, w0 = blockIdx.y * TW, tid = threadIdx.x;
+    for (int i = tid; i < TV * TW; i += 256) {
+        const int vl = i / TW, wl = i % TW;
+        uint32_t x = 0;
+        if (v0 + vl < N && w0 + wl < W) x = in[(size_t)(v0 + vl) * W + w0 + wl];
+        // vsetgts4(x, 0): byte -> 1 if > 0 (GPU_Transpose_uint8.cu:29)
+        uint32_t hd = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) hd |= (uint32_t)((int8_t)(x >> (8 * k)) > 0) << (8 * k);
+        tile[vl][wl] = hd;
+    }
+    __syncthreads();
+    const uint8_t *tb = reinterpret_cast<const uint8_t *>(&tile[0][0]);
+    const int rowb = (TW + 1) * 4;
+    for (int i = tid; i < 4 * TW * TV; i += 256) {
+        const int fl = i / TV, vl = i % TV;
+        const int v = v0 + vl, wl = fl >> 2;
+        if (v < N && w0 + wl < W) out[((size_t)(w0 * 4 + fl)) * N + v] = tb[vl * rowb + fl];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <bool EQ, bool LDS, int DMAXC>
+static hipError_t launch_decode_t(const DecodeArgs &a, int threads, hipStream_t s) {
+    const size_t lds = LDS ? (size_t)a.N * a.P * 4 : 0;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_decode<EQ, LDS, DMAXC>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)MAX_LDS_BYTES);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const unsigned nblk = (unsigned)(a.W / a.P);
+    hipLaunchKernelGGL((k_decode<EQ, LDS, DMAXC>), dim3(nblk), dim3(threads), lds, s, a);
+    return hipGetLastError();
+}
+
+template <bool EQ, bool LDS>
+static hipError_t launch_dclass(const DecodeArgs &a, int dmax, int threads, hipStream_t s) {
+    if (dmax <= 8) return launch_decode_t<EQ, LDS, 8>(a, threads, s);
+    if (dmax <= 16) return launch_decode_t<EQ, LDS, 16>(a, threads, s);
+    return launch_decode_t<EQ, LDS, 32>(a, threads, s);
+}
+
+int max_threads_for_degree(int dmax) { return dmax <= 8 ? 1024 : 256; }
+
+hipError_t launch_decode(const DecodeArgs &a, int dmax, int threads, bool post_in_lds, hipStream_t s) {
+    if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax)) return hipErrorInvalidValue;
+    const bool eq = a.variant == VAR_2NMS;
+    if (post_in_lds) return eq ? launch_dclass<true, true>(a, dmax, threads, s) : launch_dclass<false, true>(a, dmax, threads, s);
+    return eq ? launch_dclass<true, false>(a, dmax, threads, s) : launch_dclass<false, false>(a, dmax, threads, s);
+}
+
+hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipStream_t s) {
+    dim3 g((unsigned)((N + TV - 1) / TV), (unsigned)((W + TW - 1) / TW));
+    hipLaunchKernelGGL(k_interleave, g, dim3(256), 0, s, in, out, N, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s) {
+    dim3 g((unsigned)((N + TV - 1) / TV), (unsigned)((W + TW - 1) / TW));
+    hipLaunchKernelGGL(k_deinterleave_hd, g, dim3(256), 0, s, in, out, N, W);
+    return hipGetLastError();
+}
+
+}  // namespace ldpc

@@ -1,0 +1,49 @@
+// kernels.h -- launch interface between the C-ABI host code and the HIP
+// kernels of the decoding path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ldpc {
+
+// numbering shared with include/ldpc_mi355x.h
+enum Variant { VAR_MS = 0, VAR_OMS = 1, VAR_OMS_V2 = 2, VAR_NMS = 3, VAR_2NMS = 4, VAR_LOMS = 5, VAR_LNMS = 6 };
+enum LoadMode { LOAD_FRAMES = 0, LOAD_WORDS = 1, LOAD_STATE = 2 };
+
+constexpr int MAX_P = 16;                  // words per workgroup block
+constexpr int MAX_DEG = 32;                // largest check degree handled
+constexpr size_t MAX_LDS_BYTES = 160 * 1024;
+
+struct DecodeArgs {
+    // code plan (level-sorted)
+    const uint2 *checks;       // [C] x = degree | q1 << 8, y = first edge (level order)
+    const uint32_t *edge_var;  // [E] variable index per level-order edge
+    const int *level_beg;      // [n_levels + 1]
+    int n_levels, N, C, E;
+    // blocking
+    int P, log2P;              // words per workgroup (power of two)
+    size_t W;                  // words in the batch (multiple of P)
+    // state
+    uint32_t *post_state;      // [W/P][N][P] words (internal posterior)
+    uint32_t *msgs;            // [W/P][E][P] words, per check [word][j]
+    // input / output
+    const int8_t *in_frames;   // frame-major LLRs, 4W*N bytes (LOAD_FRAMES)
+    const uint32_t *in_words;  // var-major words [N][W]       (LOAD_WORDS)
+    uint8_t *out_hard;         // frame-major decisions, or null
+    int load_mode;
+    int store_state;           // write the posterior back to post_state
+    // iteration control
+    int iters, first, variant, early_term;
+    uint8_t *stopped;          // [W] per-word stop flags or null
+    int *iters_run;            // [W] iterations run in this launch, or null
+    const int8_t *ltable;      // learned table [tits][tdeg] or null
+    int tits, tdeg, iter_base;
+};
+
+hipError_t launch_decode(const DecodeArgs &a, int dmax, int threads, bool post_in_lds, hipStream_t s);
+int max_threads_for_degree(int dmax);
+hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipStream_t s);
+hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s);
+
+}  // namespace ldpc

@@ -1,0 +1,198 @@
+/*
+ * ldpc_mi355x.h -- C ABI of the MI355X-native 8-bit fixed-point Min-Sum LDPC
+ * decoder (libldpc_mi355x.so, built for gfx950).
+ *
+ * This is the drop-in boundary for the reference's GPU decoding path.  Each
+ * entry point names the reference interface it replaces (paths relative to the
+ * reference repository root).  Plain C types only; HIP streams are passed as
+ * `void *` (a hipStream_t), device pointers as plain pointers.
+ *
+ * Conventions kept from the reference:
+ *   - `nb_frames` counts 32-bit WORDS; one word packs 4 frames (frames = 4*W)
+ *     (src/main.cpp:210,214,309; src/decoder_template/CGPUDecoder.cpp:55-57).
+ *   - an LLR/posterior/message is an int8; a positive value means bit 1, so
+ *     BPSK maps bit 0 to -1 (src/awgn_channel/CChanel_AWGN_SIMD.cu:21-22) and
+ *     the hard decision is (post > 0) (src/transpose/GPU_Transpose_uint8.cu:29).
+ *   - host input is frame-major int8 in[f*N + v], f in [0, 4W); the OMS_v2
+ *     ("xMS") variant consumes it as the device layout post[v*W + w], byte k =
+ *     frame 4w+k (src/decoder_oms_v2/CGPU_Decoder_MS_SIMD_v2.cu:187).
+ *   - output is frame-major bytes out[f*N + v] in {0, 1}.
+ * Differences (documented, deliberate): errors are returned as status codes
+ * (the reference prints and calls exit(0), src/custom_api/custom_cuda.cu:5-17);
+ * iterations < 1 is rejected (the reference wraps to ~2^32 iterations,
+ * src/decoder_oms/cuda/CUDA_OMS_SIMD.cu:132,138); W need not be a multiple of
+ * 128 (src/decoder_oms/CGPU_Decoder_OMS_SIMD.cu:101-105), only of the block.
+ */
+#ifndef LDPC_MI355X_H
+#define LDPC_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+enum {
+    LDPC_OK = 0,
+    LDPC_ERR_ARG = -1,      /* invalid argument / precondition violated */
+    LDPC_ERR_HIP = -2,      /* HIP runtime error (message via ldpc_last_error) */
+    LDPC_ERR_NOMEM = -3,    /* allocation failed */
+    LDPC_ERR_NODEV = -4,    /* no usable gfx950 device */
+    LDPC_ERR_STATE = -5     /* call order (e.g. iterate before load) */
+};
+/* Text of the last error on this thread (replaces ERROR_CHECK's printf,
+ * src/custom_api/custom_cuda.cu:5-17). */
+const char *ldpc_last_error(void);
+/* Library version string. */
+const char *ldpc_version(void);
+
+/* ---- variants (one per reference decoder object) ------------------------ */
+enum {
+    LDPC_MS = 0,     /* CGPU_Decoder_MS_SIMD    src/decoder_ms/CGPU_Decoder_MS_SIMD.cu      */
+    LDPC_OMS = 1,    /* CGPU_Decoder_OMS_SIMD   src/decoder_oms/CGPU_Decoder_OMS_SIMD.cu    */
+    LDPC_OMS_V2 = 2, /* CGPU_Decoder_MS_SIMD_v2 src/decoder_oms_v2/CGPU_Decoder_MS_SIMD_v2.cu */
+    LDPC_NMS = 3,    /* CGPU_Decoder_NMS_SIMD   src/decoder_nms/CGPU_Decoder_NMS_SIMD.cu    */
+    LDPC_2NMS = 4,   /* CGPU_Decoder_2NMS_SIMD  src/decoder_2nms/CGPU_Decoder_2NMS_SIMD.cu  */
+    LDPC_LOMS = 5,   /* learned-offset OMS (int8 table [iteration][check degree])   */
+    LDPC_LNMS = 6    /* learned-normalised MS (table value a: floor(m*a/8))          */
+};
+
+/* ---- parity-check codes (replaces the compile-time tables of src/matrix:
+ * PosNoeudsVariable[_M] + DEG_1/DEG_2/DEG_*_COMPUTATIONS macros,
+ * src/matrix/4000x2000/constantes_gpu.h:6-38, constantes_decoder.h:3) ----- */
+typedef struct ldpc_code ldpc_code;
+
+/* Table form: check c (table order) has degree deg[c] and its variables are
+ * pos[off_c .. off_c+deg[c]) with off_c = sum of earlier degrees.  The
+ * serial layered decoder processes checks in exactly this order.
+ * q1_split: first table index of the reference's second degree group (OMS
+ * first-iteration no-clamp quirk, src/decoder_oms/cuda/CUDA_OMS_SIMD.cu:115-116);
+ * pass -1 to use the end of the leading run of equal degrees. */
+ldpc_code *ldpc_code_create(int n_vars, int n_checks, const int *deg,
+                            const uint32_t *pos, int q1_split);
+/* The reference's macro form: deg1 x count1 checks then deg2 x count2
+ * (src/matrix/1944x972/constantes_gpu.h:6-20). count2 may be 0. */
+ldpc_code *ldpc_code_create_grouped(int n_vars, int deg1, int count1, int deg2,
+                                    int count2, const uint32_t *pos);
+/* Parse a reference table file ("/" "* msg = i, deg = d *" "/  v0, v1, ..."),
+ * e.g. src/matrix/4000x2000/constantes_decoder.h, read from `path` at run time. */
+ldpc_code *ldpc_code_load_table(const char *path, int n_vars);
+/* Seeded generators (deterministic across platforms).
+ * Regular (dv, dc): random socket permutation, 4-cycles and repeated
+ * variables rejected (BASELINE configs 1/2/4). */
+ldpc_code *ldpc_code_regular(int n_vars, int dv, int dc, uint64_t seed);
+/* Quasi-cyclic: mb x nb base graph, circulant size z, dual-diagonal parity
+ * part on the last mb columns, info entries present with probability p,
+ * shifts uniform in [0, z-1]. */
+ldpc_code *ldpc_code_qc(int mb, int nb, int z, double p, uint64_t seed);
+/* IRA staircase code: k_info info bits, n_hi of them of degree d_hi, the rest
+ * of degree d_lo, k_info checks, parity degree 2 (staircase), info edges
+ * spread evenly over checks by a seeded permutation. */
+ldpc_code *ldpc_code_ira(int k_info, int n_hi, int d_hi, int d_lo, uint64_t seed);
+/* New code with the same H whose table order lists checks colour class by
+ * colour class (greedy colouring of the check conflict graph), so the serial
+ * schedule has few dependency levels. */
+ldpc_code *ldpc_code_reorder_by_colour(const ldpc_code *code);
+void ldpc_code_destroy(ldpc_code *code);
+
+/* n_vars, n_checks, n_edges, max check degree, dependency levels of the
+ * serial order, q1_split. Any pointer may be NULL. */
+int ldpc_code_info(const ldpc_code *code, int *n_vars, int *n_checks, int *n_edges,
+                   int *max_deg, int *n_levels, int *q1_split);
+/* Copy out the table (deg[n_checks], pos[n_edges]). */
+int ldpc_code_export(const ldpc_code *code, int *deg, uint32_t *pos);
+
+/* ---- decoder objects (replaces class CGPUDecoder,
+ * src/decoder_template/CGPUDecoder.h:19-35, and its five subclasses) ------ */
+typedef struct ldpc_decoder ldpc_decoder;
+
+typedef struct {
+    int words_per_block;  /* P: 0 = auto (power of two <= 16 dividing W)     */
+    int threads;          /* workgroup size: 0 = auto                         */
+    int early_term;       /* -1 = variant default (2NMS: on), 0 off, 1 on      */
+    int post_in_lds;      /* -1 = auto (posterior block fits LDS), 0, 1        */
+} ldpc_tuning;
+
+/* CGPUDecoder(nb_frames, n, k, m) + derived ctor: device buffers and the H
+ * table upload (src/decoder_template/CGPUDecoder.cpp:51-75).  The code is
+ * copied; `tuning` may be NULL. */
+ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
+                                  int variant, const ldpc_tuning *tuning);
+/* CGPUDecoder::initialize() (src/decoder_oms/CGPU_Decoder_OMS_SIMD.cu:92-94). */
+int ldpc_decoder_initialize(ldpc_decoder *dec);
+/* ~CGPUDecoder() (src/decoder_template/CGPUDecoder.cpp:78-91). */
+void ldpc_decoder_destroy(ldpc_decoder *dec);
+
+/* decode(var_nodes, Rprime_fix, nombre_iterations): synchronous host-buffer
+ * decode, H2D -> interleave -> iterations -> hard decision -> D2H
+ * (src/decoder_oms/CGPU_Decoder_OMS_SIMD.cu:97-149).
+ * in: 4W*N int8 (frame-major; var-major words for OMS_v2); out: 4W*N bytes. */
+int ldpc_decoder_decode(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations);
+/* Same on device buffers, enqueued on `stream` (NULL = the decoder's own).
+ * Asynchronous: returns after the launch. */
+int ldpc_decoder_decode_device(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d_out,
+                               int iterations, void *stream);
+
+/* The decoder's own HIP stream (hipStream_t) used when no stream is given. */
+void *ldpc_decoder_stream(ldpc_decoder *dec);
+/* Words per workgroup block, workgroup size, posterior-in-LDS flag and
+ * dependency levels chosen at create time. Any pointer may be NULL. */
+int ldpc_decoder_config(const ldpc_decoder *dec, int *words_per_block, int *threads,
+                        int *post_in_lds, int *n_levels);
+
+/* Learned table for LDPC_LOMS / LDPC_LNMS: int8 [n_iters][n_degs], row =
+ * absolute iteration (last row repeats), column = check degree (last column
+ * repeats).  LOMS: offset in LSB, 0..127. LNMS: numerator over 8, 0..8. */
+int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, int n_degs);
+
+/* ---- state API (per-iteration parity; the reference keeps this state in
+ * device_V / d_MSG_C_2_V, src/decoder_template/CGPUDecoder.h:21-23) ------- */
+/* Load host input (same layout as decode) and reset the decode. */
+int ldpc_decoder_load(ldpc_decoder *dec, const int8_t *in);
+/* Run `iterations` more iterations from the current state. */
+int ldpc_decoder_iterate(ldpc_decoder *dec, int iterations);
+/* Posterior in the reference layout post[v*W + w] (N*W words) and messages
+ * in the reference layout msg[e*W + w], e in table order (E*W words).
+ * Either pointer may be NULL. */
+int ldpc_decoder_get_state(ldpc_decoder *dec, uint32_t *post, uint32_t *msg);
+/* Hard decision of the current posterior, frame-major bytes (4W*N). */
+int ldpc_decoder_get_hard(ldpc_decoder *dec, uint8_t *out);
+/* Per-word stop flags (W bytes) and iterations run since load (W ints). */
+int ldpc_decoder_get_stop_info(ldpc_decoder *dec, uint8_t *stopped, int *iters_run);
+
+/* ---- device interleave (src/transpose/GPU_Transpose_uint8.cu:80-130) and
+ * inverse with hard decision (:9-78) on device buffers ------------------- */
+int ldpc_interleave_device(const int8_t *d_in, uint32_t *d_words, int n_vars,
+                           size_t nb_frames_words, void *stream);
+int ldpc_deinterleave_hd_device(const uint32_t *d_words, uint8_t *d_out, int n_vars,
+                                size_t nb_frames_words, void *stream);
+
+/* ---- device memory helpers (plumbing for callers without a HIP runtime
+ * binding; replaces CUDA_MALLOC_DEVICE/CUDA_MALLOC_HOST,
+ * src/custom_api/custom_cuda.h:24-40, and CTrame's pinned buffers,
+ * src/trame/CTrame.cpp:38-41) ------------------------------------------ */
+void *ldpc_device_alloc(size_t bytes);
+void ldpc_device_free(void *p);
+void *ldpc_host_alloc_pinned(size_t bytes);
+void ldpc_host_free_pinned(void *p);
+int ldpc_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int ldpc_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int ldpc_device_synchronize(void);
+/* Number of visible gfx950 devices (0 without a GPU). */
+int ldpc_device_count(void);
+
+/* ---- timer (replaces CTimer's cudaEvent pair, src/timer/CTimer.cu:4-91) --- */
+typedef struct ldpc_timer ldpc_timer;
+ldpc_timer *ldpc_timer_create(void *stream);
+int ldpc_timer_start(ldpc_timer *t);
+int ldpc_timer_stop(ldpc_timer *t);
+/* Elapsed milliseconds between start and stop (synchronises on stop). */
+double ldpc_timer_elapsed_ms(ldpc_timer *t);
+void ldpc_timer_destroy(ldpc_timer *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

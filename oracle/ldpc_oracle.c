@@ -1,0 +1,429 @@
+/*
+ * ldpc_oracle.c -- CPU restatement of the reference's integer Min-Sum path.
+ *
+ * TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+ * the cpu_baseline leg of bench.py as the checker; the shipped HIP library
+ * never links or calls it.
+ *
+ * Structure deliberately follows the reference's per-thread kernel: one
+ * uint32_t word carries 4 frames (one int8 lane each) and the checks are
+ * processed serially in table order, posterior updated in place after every
+ * check.  The per-byte operations are written from their mathematical
+ * definitions (src/decoder_ms/cuda/simd_functions.h:89-131), byte by byte
+ * with plain int arithmetic.
+ */
+#include "ldpc_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* per-byte ops                                                              */
+/* ------------------------------------------------------------------------ */
+static inline int sb(uint32_t x, int k) { return (int8_t)((x >> (8 * k)) & 0xFFu); }
+static inline int ub(uint32_t x, int k) { return (int)((x >> (8 * k)) & 0xFFu); }
+static inline uint32_t put(uint32_t acc, int k, int v) { return acc | ((uint32_t)(v & 0xFF) << (8 * k)); }
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* vaddss4: per-byte signed saturating add (simd_functions.h:96) */
+uint32_t orc_vaddss4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) r = put(r, k, clampi(sb(a, k) + sb(b, k), -128, 127));
+    return r;
+}
+/* vsubss4: per-byte signed saturating subtract (simd_functions.h:129) */
+uint32_t orc_vsubss4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) r = put(r, k, clampi(sb(a, k) - sb(b, k), -128, 127));
+    return r;
+}
+/* vsubus4: per-byte unsigned saturating subtract (simd_functions.h:130) */
+uint32_t orc_vsubus4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) r = put(r, k, clampi(ub(a, k) - ub(b, k), 0, 255));
+    return r;
+}
+/* vminu4 / vmaxu4: per-byte unsigned min / max (simd_functions.h:111,109) */
+uint32_t orc_vminu4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) { int x = ub(a, k), y = ub(b, k); r = put(r, k, x < y ? x : y); }
+    return r;
+}
+uint32_t orc_vmaxu4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) { int x = ub(a, k), y = ub(b, k); r = put(r, k, x > y ? x : y); }
+    return r;
+}
+/* vabs4: per-byte |a| with wrap-around, |-128| = 0x80 (simd_functions.h:91) */
+uint32_t orc_vabs4(uint32_t a) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) { int x = sb(a, k); r = put(r, k, x < 0 ? -x : x); }
+    return r;
+}
+/* vneg4: per-byte wrap-around negation (simd_functions.h:112) */
+uint32_t orc_vneg4(uint32_t a) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) r = put(r, k, -sb(a, k));
+    return r;
+}
+/* vcmpgts4: per-byte signed a > b ? 0xff : 0 (simd_functions.h:102) */
+uint32_t orc_vcmpgts4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) r = put(r, k, sb(a, k) > sb(b, k) ? 0xFF : 0);
+    return r;
+}
+/* vcmpeq4 / vcmpne4: per-byte equality masks (simd_functions.h:99,108) */
+uint32_t orc_vcmpeq4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) r = put(r, k, ub(a, k) == ub(b, k) ? 0xFF : 0);
+    return r;
+}
+uint32_t orc_vcmpne4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) r = put(r, k, ub(a, k) != ub(b, k) ? 0xFF : 0);
+    return r;
+}
+/* vsetgts4: per-byte signed a > b ? 1 : 0 (simd_functions.h:122) */
+uint32_t orc_vsetgts4(uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) r = put(r, k, sb(a, k) > sb(b, k) ? 1 : 0);
+    return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* magnitude functions per variant                                           */
+/* ------------------------------------------------------------------------ */
+/* NMS / 2NMS scale each byte through a float multiply and truncation to char:
+ * src/decoder_nms/cuda/CUDA_NMS_SIMD.cu:73-85 (0.75),
+ * src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:80-83 (0.875 on min2). */
+static uint32_t scale_bytes(uint32_t m, float f) {
+    uint32_t r = 0;
+    for (int k = 0; k < 4; k++) {
+        signed char c = (signed char)sb(m, k);
+        signed char s = (signed char)(((float)c) * f);
+        r = put(r, k, s);
+    }
+    return r;
+}
+
+/* cste_1 (value given to the min edge, from min2) and cste_2 (others, from
+ * min1) for one check.  `nocap` = OMS first-iteration second degree group
+ * (src/decoder_oms/cuda/CUDA_OMS_SIMD.cu:115-116). */
+static void constants(int variant, uint32_t min1, uint32_t min2, int nocap,
+                      int lparam, uint32_t *c1, uint32_t *c2) {
+    switch (variant) {
+    case ORC_MS: /* CUDA_MS_SIMD.cu:73-74 */
+        *c1 = orc_vminu4(min2, 0x1F1F1F1Fu);
+        *c2 = orc_vminu4(min1, 0x1F1F1F1Fu);
+        break;
+    case ORC_OMS: /* CUDA_OMS_SIMD.cu:73-74, 115-116 */
+        if (nocap) {
+            *c1 = orc_vsubus4(min2, 0x01010101u);
+            *c2 = orc_vsubus4(min1, 0x01010101u);
+        } else {
+            *c1 = orc_vminu4(orc_vsubus4(min2, 0x01010101u), 0x1F1F1F1Fu);
+            *c2 = orc_vminu4(orc_vsubus4(min1, 0x01010101u), 0x1F1F1F1Fu);
+        }
+        break;
+    case ORC_OMS_V2: /* CUDA_OMS_SIMD_v2.cu:82-83, 145-146 (clamp everywhere) */
+        *c1 = orc_vminu4(orc_vsubus4(min2, 0x01010101u), 0x1F1F1F1Fu);
+        *c2 = orc_vminu4(orc_vsubus4(min1, 0x01010101u), 0x1F1F1F1Fu);
+        break;
+    case ORC_NMS: /* CUDA_NMS_SIMD.cu:73-85 */
+        *c1 = scale_bytes(min2, 0.750f);
+        *c2 = scale_bytes(min1, 0.750f);
+        break;
+    case ORC_2NMS: /* CUDA_2NMS_SIMD.cu:73-85 */
+        *c1 = scale_bytes(min2, 0.875f);
+        *c2 = scale_bytes(min1, 0.750f);
+        break;
+    case ORC_LOMS: { /* learned offset: min(sat_u(m - beta), 31) */
+        uint32_t b4 = (uint32_t)(lparam & 0xFF) * 0x01010101u;
+        *c1 = orc_vminu4(orc_vsubus4(min2, b4), 0x1F1F1F1Fu);
+        *c2 = orc_vminu4(orc_vsubus4(min1, b4), 0x1F1F1F1Fu);
+        break;
+    }
+    case ORC_LNMS: { /* learned normalisation: floor(m * a / 8), a in [0,8] */
+        uint32_t r1 = 0, r2 = 0;
+        for (int k = 0; k < 4; k++) {
+            r1 = put(r1, k, (ub(min2, k) * lparam) >> 3);
+            r2 = put(r2, k, (ub(min1, k) * lparam) >> 3);
+        }
+        *c1 = r1;
+        *c2 = r2;
+        break;
+    }
+    default:
+        *c1 = *c2 = 0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* one check on one word: CUDA_OMS_SIMD.cu:148-191 (iterations >= 2) and     */
+/* :47-88 (first iteration, no message read)                                 */
+/* ------------------------------------------------------------------------ */
+#define ORC_MAXD 64
+static uint32_t check_word(int d, const uint32_t *vars, uint32_t *post, uint32_t *msg,
+                           size_t W, size_t w, int first, int variant, int nocap,
+                           int lparam) {
+    uint32_t t[ORC_MAXD];
+    uint32_t sign = 0, min1 = 0x7F7F7F7Fu, min2 = 0x7F7F7F7Fu;
+    for (int j = 0; j < d; j++) {
+        uint32_t p = post[(size_t)vars[j] * W + w];
+        t[j] = first ? p : orc_vsubss4(p, msg[(size_t)j * W + w]);      /* :166 */
+        uint32_t a = orc_vabs4(t[j]);
+        min2 = orc_vminu4(min2, orc_vmaxu4(a, min1));                    /* :168 */
+        min1 = orc_vminu4(min1, a);                                      /* :169 */
+        sign ^= orc_vcmpgts4(t[j], 0);                                   /* :170 */
+    }
+    uint32_t c1, c2;
+    constants(variant, min1, min2, nocap, lparam, &c1, &c2);             /* :173-174 */
+    for (int j = 0; j < d; j++) {
+        uint32_t ab = orc_vabs4(t[j]);
+        uint32_t m1 = orc_vcmpeq4(ab, min1);
+        uint32_t m2 = orc_vcmpne4(ab, min1);
+        uint32_t re = (m1 & c1) | (m2 & c2);                             /* :181 */
+        uint32_t sm = sign ^ orc_vcmpgts4(t[j], 0);                      /* :182 */
+        uint32_t out = (re & sm) | (orc_vneg4(re) & ~sm);                /* :183 */
+        msg[(size_t)j * W + w] = out;                                    /* :184 */
+        post[(size_t)vars[j] * W + w] = orc_vaddss4(t[j], out);          /* :186 */
+    }
+    return sign;
+}
+
+static int table_param(int variant, const int8_t *table, int tits, int tdeg,
+                       int abs_it, int d) {
+    if (variant != ORC_LOMS && variant != ORC_LNMS) return 0;
+    int row = abs_it < tits ? abs_it : tits - 1;
+    int col = d < tdeg ? d : tdeg - 1;
+    return table[(size_t)row * tdeg + col];
+}
+
+/* Decode words [w0, w1). */
+static void decode_range(int N, int C, const int *deg, const uint32_t *pos,
+                         const size_t *eoff, int variant, size_t W, uint32_t *post,
+                         uint32_t *msg, int iters, int first, int q1_split,
+                         int early_term, uint8_t *stopped, const int8_t *table,
+                         int tits, int tdeg, int iter_base, int *iters_run,
+                         size_t w0, size_t w1) {
+    (void)N;
+    for (size_t w = w0; w < w1; w++) {
+        int ran = 0;
+        if (stopped && stopped[w]) { if (iters_run) iters_run[w] = 0; continue; }
+        for (int it = 0; it < iters; it++) {
+            int is_first = first && it == 0;
+            uint32_t ov = 0;
+            for (int c = 0; c < C; c++) {
+                int d = deg[c];
+                int nocap = is_first && variant == ORC_OMS && c >= q1_split;
+                int lp = table_param(variant, table, tits, tdeg, iter_base + it, d);
+                ov |= check_word(d, pos + eoff[c], post, msg + eoff[c] * W, W, w,
+                                 is_first, variant, nocap, lp);
+            }
+            ran++;
+            /* CUDA_2NMS_SIMD.cu:237-244: stop test after every iteration of the
+             * while loop, i.e. every iteration except a decode's first one. */
+            if (early_term && !is_first && ov == 0) {
+                if (stopped) stopped[w] = 1;
+                break;
+            }
+        }
+        if (iters_run) iters_run[w] = ran;
+    }
+}
+
+static int validate(int N, int C, const int *deg, const uint32_t *pos, int variant,
+                    int iters, const int8_t *table, int tits, int tdeg, size_t **eoff_out) {
+    if (N <= 0 || C <= 0 || !deg || !pos || iters < 1) return -1;
+    if (variant < ORC_MS || variant > ORC_LNMS) return -1;
+    if ((variant == ORC_LOMS || variant == ORC_LNMS) && (!table || tits < 1 || tdeg < 1)) return -1;
+    size_t *eoff = (size_t *)malloc(sizeof(size_t) * (size_t)(C + 1));
+    if (!eoff) return -1;
+    eoff[0] = 0;
+    for (int c = 0; c < C; c++) {
+        if (deg[c] < 1 || deg[c] > ORC_MAXD) { free(eoff); return -1; }
+        eoff[c + 1] = eoff[c] + (size_t)deg[c];
+    }
+    for (size_t e = 0; e < eoff[C]; e++)
+        if (pos[e] >= (uint32_t)N) { free(eoff); return -1; }
+    *eoff_out = eoff;
+    return 0;
+}
+
+int orc_decode(int N, int C, const int *deg, const uint32_t *pos, int variant,
+               size_t W, uint32_t *post, uint32_t *msg, int iters, int first,
+               int q1_split, int early_term, uint8_t *stopped,
+               const int8_t *table, int tits, int tdeg, int iter_base,
+               int *iters_run) {
+    size_t *eoff;
+    if (validate(N, C, deg, pos, variant, iters, table, tits, tdeg, &eoff)) return -1;
+    decode_range(N, C, deg, pos, eoff, variant, W, post, msg, iters, first, q1_split,
+                 early_term, stopped, table, tits, tdeg, iter_base, iters_run, 0, W);
+    free(eoff);
+    return 0;
+}
+
+typedef struct {
+    int N, C; const int *deg; const uint32_t *pos; const size_t *eoff; int variant;
+    size_t W; uint32_t *post; uint32_t *msg; int iters, first, q1_split, early_term;
+    uint8_t *stopped; const int8_t *table; int tits, tdeg, iter_base; int *iters_run;
+    size_t w0, w1;
+} job_t;
+
+static void *job_run(void *p) {
+    job_t *j = (job_t *)p;
+    decode_range(j->N, j->C, j->deg, j->pos, j->eoff, j->variant, j->W, j->post, j->msg,
+                 j->iters, j->first, j->q1_split, j->early_term, j->stopped, j->table,
+                 j->tits, j->tdeg, j->iter_base, j->iters_run, j->w0, j->w1);
+    return NULL;
+}
+
+int orc_decode_mt(int N, int C, const int *deg, const uint32_t *pos, int variant,
+                  size_t W, uint32_t *post, uint32_t *msg, int iters, int first,
+                  int q1_split, int early_term, uint8_t *stopped,
+                  const int8_t *table, int tits, int tdeg, int iter_base,
+                  int *iters_run, int threads) {
+    size_t *eoff;
+    if (validate(N, C, deg, pos, variant, iters, table, tits, tdeg, &eoff)) return -1;
+    if (threads < 1) threads = 1;
+    if ((size_t)threads > W) threads = (int)(W ? W : 1);
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    job_t *jobs = (job_t *)malloc(sizeof(job_t) * (size_t)threads);
+    for (int t = 0; t < threads; t++) {
+        job_t j = {N, C, deg, pos, eoff, variant, W, post, msg, iters, first, q1_split,
+                   early_term, stopped, table, tits, tdeg, iter_base, iters_run,
+                   W * (size_t)t / (size_t)threads, W * (size_t)(t + 1) / (size_t)threads};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, job_run, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+    free(eoff);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* layouts: closed form of src/transpose/GPU_Transpose_uint8.cu:80-130       */
+/* (Interleaver_uint8) and :9-78 (InvInterleaver_uint8 + vsetgts4 at :29)    */
+/* ------------------------------------------------------------------------ */
+void orc_interleave(const int8_t *in, uint32_t *post, int N, size_t W) {
+    for (int v = 0; v < N; v++)
+        for (size_t w = 0; w < W; w++) {
+            uint32_t x = 0;
+            for (int k = 0; k < 4; k++) x = put(x, k, in[(4 * w + (size_t)k) * (size_t)N + (size_t)v]);
+            post[(size_t)v * W + w] = x;
+        }
+}
+
+void orc_deinterleave_hd(const uint32_t *post, uint8_t *out, int N, size_t W) {
+    for (int v = 0; v < N; v++)
+        for (size_t w = 0; w < W; w++) {
+            uint32_t hd = orc_vsetgts4(post[(size_t)v * W + w], 0);
+            for (int k = 0; k < 4; k++)
+                out[(4 * w + (size_t)k) * (size_t)N + (size_t)v] = (uint8_t)ub(hd, k);
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* channel                                                                   */
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers:
+ * as easy as 1, 2, 3"): multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key
+ * increments 0x9E3779B9 / 0xBB67AE85, ten rounds.                            */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n1 = lo1;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        uint32_t n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* src/awgn_channel/CChanel_AWGN_SIMD.cu:63-73: sigma = sqrt(10^(-(EbN0+10log10 R)/10)/2) */
+double orc_sigma(double ebn0_db, double rate) {
+    double interm = -0.1 * (ebn0_db + 10.0 * log10(rate));
+    return sqrt(pow(10.0, interm) / 2.0);
+}
+
+/* Box-Muller (cosine branch) on two 53-bit uniforms from one Philox block:
+ * counter = (s_lo, s_hi, snr_idx, 0x4C445043), key = (seed_lo, seed_hi).   */
+double orc_gauss(uint64_t seed, uint32_t snr_idx, uint64_t s) {
+    uint32_t ctr[4] = {(uint32_t)s, (uint32_t)(s >> 32), snr_idx, 0x4C445043u};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t r[4];
+    orc_philox4x32_10(ctr, key, r);
+    uint64_t a = (((uint64_t)r[0] << 32) | r[1]) >> 11;
+    uint64_t b = (((uint64_t)r[2] << 32) | r[3]) >> 11;
+    double u1 = ((double)a + 1.0) * 0x1.0p-53; /* (0, 1] */
+    double u2 = (double)b * 0x1.0p-53;         /* [0, 1) */
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925286766559 * u2);
+}
+
+int8_t orc_llr(uint64_t seed, uint32_t snr_idx, uint64_t s, double sigma, int quant) {
+    double y = -1.0 + sigma * orc_gauss(seed, snr_idx, s);
+    if (quant == 1) {
+        /* CChanel_AWGN_SIMD.cu:21-24: (char)fminf(fmaxf(8*v, -31), 31) */
+        float v = (float)y;
+        float q = fminf(fmaxf(8.0f * v, -31.0f), 31.0f);
+        return (int8_t)q;
+    }
+    double q = round(16.0 * y / (sigma * sigma)); /* 8 * (2y / sigma^2) */
+    if (q > 127.0) q = 127.0;
+    if (q < -127.0) q = -127.0;
+    return (int8_t)q;
+}
+
+void orc_channel_frames(int8_t *out, int N, size_t f0, size_t nf, uint64_t seed,
+                        uint32_t snr_idx, double sigma, int quant) {
+    for (size_t f = 0; f < nf; f++)
+        for (int v = 0; v < N; v++)
+            out[f * (size_t)N + (size_t)v] =
+                orc_llr(seed, snr_idx, (f0 + f) * (uint64_t)N + (uint64_t)v, sigma, quant);
+}
+
+/* ------------------------------------------------------------------------ */
+/* error counting: src/ber_analyzer/CErrorAnalyzer.cpp:141-157 (counts the   */
+/* first _vars = N - K positions of each frame, all-zero codeword)           */
+/* ------------------------------------------------------------------------ */
+long long orc_count_errors(const uint8_t *hard, int N, size_t frames, int n_info,
+                           long long *bit_errors, uint32_t *per_frame_bits) {
+    long long fe = 0, be = 0;
+    for (size_t f = 0; f < frames; f++) {
+        uint32_t e = 0;
+        for (int v = 0; v < n_info; v++) e += hard[f * (size_t)N + (size_t)v] != 0;
+        be += e;
+        fe += e != 0;
+        if (per_frame_bits) per_frame_bits[f] = e;
+    }
+    if (bit_errors) *bit_errors = be;
+    return fe;
+}
+
+void orc_syndrome(const uint8_t *hard, int N, int C, const int *deg,
+                  const uint32_t *pos, size_t frames, uint8_t *ok) {
+    for (size_t f = 0; f < frames; f++) {
+        const uint8_t *h = hard + f * (size_t)N;
+        size_t e = 0;
+        int good = 1;
+        for (int c = 0; c < C; c++) {
+            int par = 0;
+            for (int j = 0; j < deg[c]; j++) par ^= h[pos[e + (size_t)j]] & 1;
+            e += (size_t)deg[c];
+            if (par) good = 0;
+        }
+        ok[f] = (uint8_t)good;
+    }
+}

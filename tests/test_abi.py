@@ -1,0 +1,60 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol declared in
+include/ldpc_mi355x.h, the Python binding covers exactly those symbols, and
+argument validation that needs no GPU behaves."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import fast_ldpc_decoder_for_gpu_fixed_amd as L
+from fast_ldpc_decoder_for_gpu_fixed_amd import _lib
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "ldpc_mi355x.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(ldpc_[a-z0-9_]+)\s*\(", text))
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    names = declared_functions()
+    assert len(names) > 30
+    lib = L.load()
+    for n in sorted(names):
+        assert hasattr(lib, n), f"{n} declared but not exported"
+    assert names == set(_lib.SIGNATURES), (names ^ set(_lib.SIGNATURES))
+
+
+def test_version_and_no_device_here():
+    lib = L.load()
+    assert b"gfx950" in lib.ldpc_version()
+    assert L.device_count() >= 0
+
+
+def test_code_validation_errors():
+    with pytest.raises(L.LdpcError):
+        L.Code.from_table(4, np.array([2, 2], np.int32), np.array([0, 1, 2, 9], np.uint32))  # var out of range
+    with pytest.raises(L.LdpcError):
+        L.Code.regular(10, 3, 4, 1)  # n*dv not a multiple of dc
+    with pytest.raises(L.LdpcError):
+        L.Code.load_table("/nonexistent/table.h")
+
+
+def test_degree_above_limit_rejected_before_device_use():
+    code = L.Code.from_table(40, np.array([33], np.int32), np.arange(33, dtype=np.uint32))
+    with pytest.raises(L.LdpcError, match="degree"):
+        L.make_decoder(code, 4, L.OMS)
+
+
+def test_grouped_form_matches_table_form():
+    rng = np.random.default_rng(1)
+    pos = np.concatenate([rng.choice(50, 5, replace=False) for _ in range(6)]
+                         + [rng.choice(50, 4, replace=False) for _ in range(4)]).astype(np.uint32)
+    g = L.Code.grouped(50, 5, 6, 4, 4, pos)
+    t = L.Code.from_table(50, np.array([5] * 6 + [4] * 4, np.int32), pos)
+    assert g.q1_split == 6 == t.q1_split
+    for a, b in zip(g.export(), t.export()):
+        np.testing.assert_array_equal(a, b)
