@@ -418,7 +418,7 @@ static hipError_t launch_decode_t(const DecodeArgs &a, int threads, hipStream_t 
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)k_decode<EQ, LDS, DMAXC>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)MAX_LDS_BYTES);
+                                           (int)(MAX_LDS_BYTES - 1024));  // static flags live in the rest
         if (e != hipSuccess) return e;
         attr_set = true;
     }

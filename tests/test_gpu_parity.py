@@ -21,7 +21,7 @@ def channel_llr(n, frames, ebn0=2.0, rate=0.5, seed=0, quant=0):
     """Host test input: all-zero codeword, BPSK bit0 -> -1, AWGN."""
     rng = np.random.default_rng(seed)
     sigma = np.sqrt(1.0 / (2 * rate * 10 ** (ebn0 / 10)))
-    y = -1.0 + sigma * rng.standard_normal(frames * n)
+    y = -1.0 + sigma * rng.standard_normal(frames * n, dtype=np.float32)
     if quant == 0:
         q = np.clip(np.round(16.0 * y / sigma ** 2), -127, 127)
     else:
@@ -119,16 +119,18 @@ def test_2nms_early_term_lockstep(gpu_available):
     EARLY_TERM=1, src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:17)."""
     code = code36()
     W = 32
-    llr = channel_llr(4000, 4 * W, ebn0=3.0, seed=9)
+    # reference quantizer at 2.5 dB: some words stop at iteration 6-7, some never
+    llr = channel_llr(4000, 4 * W, ebn0=2.5, seed=9, quant=1)
     dec, orc = run_lockstep(code, W, L.NMS2, llr, 20, early_term=1)
-    assert orc.stopped.any()
+    assert orc.stopped.any() and not orc.stopped.all()
 
 
 def test_oms_early_term_lockstep(gpu_available):
     code = code36()
     W = 32
     llr = channel_llr(4000, 4 * W, ebn0=2.5, seed=19)
-    run_lockstep(code, W, L.OMS, llr, 15, early_term=1, check_every=3)
+    _, orc = run_lockstep(code, W, L.OMS, llr, 15, early_term=1, check_every=3)
+    assert orc.stopped.any()
 
 
 def test_unordered_table_many_levels(gpu_available):
@@ -198,7 +200,9 @@ def test_full_size_batch_sampled(variant, gpu_available):
     deg, pos = code.export()
     W = 16384
     N = 4000
-    llr = channel_llr(N, 4 * W, ebn0=2.0, seed=2024 + variant)
+    if "full" not in _CODES:
+        _CODES["full"] = channel_llr(N, 4 * W, ebn0=2.0, seed=2024)
+    llr = _CODES["full"]
     dec = L.make_decoder(code, W, variant, early_term=0)
     x = O.interleave(llr, N).view(np.int8) if variant == L.OMS_V2 else llr
     out = dec.decode_bytes(x, 20)
