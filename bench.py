@@ -34,9 +34,11 @@ def parse():
     ap.add_argument("--ebn0", type=float, default=2.0)
     ap.add_argument("--words-per-block", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--cpu-baseline-words", type=int, default=256)
+    ap.add_argument("--cpu-baseline-words", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "hbm_traffic.json"))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -74,6 +76,13 @@ def main():
               f"across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     import fast_ldpc_decoder_for_gpu_fixed_amd as L
+
+    if args.copy_calibration:
+        a, b = L.DeviceBuffer(args.copy_calibration), L.DeviceBuffer(args.copy_calibration)
+        L.check(L.load().ldpc_memcpy_d2d(L.C.c_void_p(b.ptr), L.C.c_void_p(a.ptr), args.copy_calibration), "d2d")
+        L.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_calibration}))
+        return
 
     variant = {"MS": L.MS, "OMS": L.OMS, "OMS_V2": L.OMS_V2, "NMS": L.NMS, "2NMS": L.NMS2}[args.variant.upper()]
     code = L.code_3_6(4000, seed=1)
@@ -139,6 +148,7 @@ def main():
                                "%d frames resident in HBM" % (L.VARIANT_NAMES[variant], args.iters, frames),
                    "frames": frames, "iterations": args.iters, "variant": L.VARIANT_NAMES[variant],
                    "words_per_block": cfg["words_per_block"], "threads": cfg["threads"],
+                   "single_degree_kernel": cfg["uniform_deg"] > 0, "batch": cfg["batch"],
                    "levels_per_iteration": cfg["n_levels"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

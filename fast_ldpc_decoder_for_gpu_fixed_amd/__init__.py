@@ -122,7 +122,8 @@ class CGPUDecoder:
 
     def __init__(self, nb_frames: int, n: int, k: int, m: int, code: Code,
                  words_per_block: int = 0, threads: int = 0, early_term: int = -1,
-                 post_in_lds: int = -1, variant: Optional[int] = None):
+                 post_in_lds: int = -1, uniform: int = -1, batch: int = 0,
+                 variant: Optional[int] = None):
         if (n, k, m) != (code.n_vars, code.n_checks, code.n_edges):
             raise LdpcError(f"(n, k, m) = {(n, k, m)} does not match the code "
                             f"{(code.n_vars, code.n_checks, code.n_edges)}")
@@ -132,7 +133,7 @@ class CGPUDecoder:
         self.code = code
         self.nb_frames = nb_frames
         self.n, self.k, self.m = n, k, m
-        tu = Tuning(words_per_block, threads, early_term, post_in_lds)
+        tu = Tuning(words_per_block, threads, early_term, post_in_lds, uniform, batch)
         h = load().ldpc_decoder_create(code.handle, nb_frames, self.variant, C.byref(tu))
         if not h:
             raise LdpcError(f"ldpc_decoder_create failed: {_lib.last_error()}")
@@ -184,10 +185,11 @@ class CGPUDecoder:
         return int(load().ldpc_decoder_stream(self._h) or 0)
 
     def config(self) -> dict:
-        p, t, l, lv = (C.c_int() for _ in range(4))
-        check(load().ldpc_decoder_config(self._h, C.byref(p), C.byref(t), C.byref(l), C.byref(lv)), "config")
+        p, t, l, lv, u, b = (C.c_int() for _ in range(6))
+        check(load().ldpc_decoder_config(self._h, C.byref(p), C.byref(t), C.byref(l), C.byref(lv), C.byref(u),
+                                         C.byref(b)), "config")
         return {"words_per_block": p.value, "threads": t.value, "post_in_lds": bool(l.value),
-                "n_levels": lv.value}
+                "n_levels": lv.value, "uniform_deg": u.value, "batch": b.value}
 
     def set_table(self, table) -> None:
         t = np.ascontiguousarray(table, dtype=np.int8)

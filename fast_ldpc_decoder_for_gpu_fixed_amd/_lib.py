@@ -17,7 +17,8 @@ _u8_p = C.POINTER(C.c_uint8)
 
 class Tuning(C.Structure):
     _fields_ = [("words_per_block", C.c_int), ("threads", C.c_int),
-                ("early_term", C.c_int), ("post_in_lds", C.c_int)]
+                ("early_term", C.c_int), ("post_in_lds", C.c_int), ("uniform", C.c_int),
+                ("batch", C.c_int)]
 
 
 # name -> (restype, argtypes); mirrors include/ldpc_mi355x.h one to one
@@ -40,7 +41,7 @@ SIGNATURES = {
     "ldpc_decoder_decode": (C.c_int, [C.c_void_p, _i8_p, _u8_p, C.c_int]),
     "ldpc_decoder_decode_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ldpc_decoder_stream": (C.c_void_p, [C.c_void_p]),
-    "ldpc_decoder_config": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
+    "ldpc_decoder_config": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "ldpc_decoder_set_table": (C.c_int, [C.c_void_p, _i8_p, C.c_int, C.c_int]),
     "ldpc_decoder_load": (C.c_int, [C.c_void_p, _i8_p]),
     "ldpc_decoder_iterate": (C.c_int, [C.c_void_p, C.c_int]),
@@ -55,6 +56,7 @@ SIGNATURES = {
     "ldpc_host_free_pinned": (None, [C.c_void_p]),
     "ldpc_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "ldpc_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "ldpc_memcpy_d2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "ldpc_device_synchronize": (C.c_int, []),
     "ldpc_device_count": (C.c_int, []),
     "ldpc_timer_create": (C.c_void_p, [C.c_void_p]),

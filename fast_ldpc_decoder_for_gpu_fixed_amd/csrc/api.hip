@@ -64,11 +64,13 @@ struct ldpc_decoder {
     size_t W = 0;
     int variant = 0;
     int P = 1, log2P = 0, threads = 256;
+    int uniform_deg = 0, batch = 1;  // single-degree kernel when uniform_deg > 0
     bool post_in_lds = true;
     int early_term = 0;
     hipStream_t stream = nullptr;
     // device
     uint2 *d_checks = nullptr;
+    uint32_t *d_edge_off16 = nullptr;
     uint32_t *d_edge_var = nullptr;
     int *d_level_beg = nullptr;
     uint32_t *d_post = nullptr;
@@ -103,6 +105,7 @@ void free_decoder(ldpc_decoder *d) {
     hipFree(d->d_out);
     hipFree(d->d_stopped);
     hipFree(d->d_iters);
+    hipFree(d->d_edge_off16);
     hipFree(d->d_table);
     if (d->stream) hipStreamDestroy(d->stream);
     delete d;
@@ -120,6 +123,7 @@ ldpc::DecodeArgs base_args(const ldpc_decoder *d) {
     std::memset(&a, 0, sizeof a);
     a.checks = d->d_checks;
     a.edge_var = d->d_edge_var;
+    a.edge_off16 = d->d_edge_off16;
     a.level_beg = d->d_level_beg;
     a.n_levels = d->plan.n_levels;
     a.N = d->code.N;
@@ -145,7 +149,7 @@ ldpc::DecodeArgs base_args(const ldpc_decoder *d) {
 int launch(ldpc_decoder *d, const ldpc::DecodeArgs &a, hipStream_t s) {
     if ((d->variant == LDPC_LOMS || d->variant == LDPC_LNMS) && !d->d_table)
         return fail(LDPC_ERR_STATE, "learned variant needs ldpc_decoder_set_table first");
-    HIP_TRY(ldpc::launch_decode(a, d->plan.max_deg, d->threads, d->post_in_lds, s));
+    HIP_TRY(ldpc::launch_decode(a, d->plan.max_deg, d->threads, d->post_in_lds, s, d->uniform_deg, d->batch));
     return LDPC_OK;
 }
 
@@ -269,7 +273,7 @@ ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
         fail(LDPC_ERR_NODEV, why);
         return nullptr;
     }
-    ldpc_tuning tu = {0, 0, -1, -1};
+    ldpc_tuning tu = {0, 0, -1, -1, -1, 0};
     if (tuning) tu = *tuning;
     ldpc_decoder *d = new ldpc_decoder();
     d->code = *code;
@@ -288,7 +292,7 @@ ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
     int P = tu.words_per_block;
     if (P <= 0) {
         P = 1;
-        const size_t target = d->post_in_lds ? 80 * 1024 : (size_t)1 << 30;
+        const size_t target = d->post_in_lds ? 40 * 1024 : (size_t)1 << 30;  // ~4 workgroups per CU
         while (P * 2 <= ldpc::MAX_P && post_word_bytes * (size_t)(P * 2) <= target && nb_frames_words % (size_t)(P * 2) == 0)
             P *= 2;
     }
@@ -301,13 +305,51 @@ ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
     d->P = P;
     while ((1 << d->log2P) < P) d->log2P++;
     int T = tu.threads > 0 ? tu.threads : 256;
-    T = std::min(T, ldpc::max_threads_for_degree(d->plan.max_deg));
-    if (T % 64 || T % P || T < 64) {
+    if (T % 64 || T % P || T < 64 || T > 1024) {
         free_decoder(d);
-        fail(LDPC_ERR_ARG, "threads must be a multiple of 64 and of words_per_block");
+        fail(LDPC_ERR_ARG, "threads must be a multiple of 64 and of words_per_block, at most 1024");
         return nullptr;
     }
     d->threads = T;
+
+    // single-degree kernel: every check of degree D, LDS offsets fit 16 bits,
+    // and no check is subject to the OMS first-iteration no-clamp group
+    int dmin = ldpc::MAX_DEG + 1;
+    for (int x : code->deg) dmin = std::min(dmin, x);
+    const bool uni_ok = dmin == code->max_deg && ldpc::uniform_degree_supported(dmin) && d->post_in_lds &&
+                        (size_t)code->N * (size_t)P * 4 <= 65535 && (variant != LDPC_OMS || code->q1_split >= code->C);
+    if (tu.uniform == 1 && !uni_ok) {
+        free_decoder(d);
+        fail(LDPC_ERR_ARG, "single-degree kernel requested but the code/blocking does not qualify");
+        return nullptr;
+    }
+    d->uniform_deg = (tu.uniform != 0 && uni_ok) ? dmin : 0;
+    if (T > ldpc::max_threads_for_degree(code->max_deg, d->uniform_deg > 0)) {
+        if (tu.threads > 0) {
+            free_decoder(d);
+            fail(LDPC_ERR_ARG, "threads above the kernel's limit for this code (" +
+                                   std::to_string(ldpc::max_threads_for_degree(code->max_deg, d->uniform_deg > 0)) + ")");
+            return nullptr;
+        }
+        T = ldpc::max_threads_for_degree(code->max_deg, d->uniform_deg > 0);
+    }
+    d->threads = T;
+    d->batch = tu.batch > 0 ? tu.batch : 2;
+    if (d->batch != 1 && d->batch != 2 && d->batch != 4) {
+        free_decoder(d);
+        fail(LDPC_ERR_ARG, "batch must be 1, 2 or 4");
+        return nullptr;
+    }
+    std::vector<uint32_t> off16;
+    if (d->uniform_deg) {
+        const int D = d->uniform_deg, H = (D + 1) / 2;
+        off16.assign((size_t)code->C * H, 0);
+        for (int cs = 0; cs < code->C; cs++)
+            for (int j = 0; j < D; j++) {
+                const uint32_t o = (d->plan.edge_var[(size_t)cs * D + j] << d->log2P) * 4;  // LDS byte offset
+                off16[(size_t)cs * H + j / 2] |= (j & 1) ? (o << 16) : o;
+            }
+    }
 
     const size_t W = d->W;
     const ldpc_code &c = d->code;
@@ -324,10 +366,13 @@ ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
     ALLOC(d_out, (size_t)c.N * W * 4);
     ALLOC(d_stopped, W);
     ALLOC(d_iters, W);
+    ALLOC(d_edge_off16, off16.size());
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMemcpy(d->d_checks, ck.data(), ck.size() * sizeof(uint2), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d->d_edge_var, d->plan.edge_var.data(), (size_t)c.E * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !off16.empty())
+        e = hipMemcpy(d->d_edge_off16, off16.data(), off16.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d->d_level_beg, d->plan.level_beg.data(), d->plan.level_beg.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(d->d_msgs, 0, (size_t)c.E * W * 4);
     if (e == hipSuccess) e = hipMemset(d->d_stopped, 0, W);
@@ -355,12 +400,14 @@ void ldpc_decoder_destroy(ldpc_decoder *dec) {
 void *ldpc_decoder_stream(ldpc_decoder *dec) { return dec ? (void *)dec->stream : nullptr; }
 
 int ldpc_decoder_config(const ldpc_decoder *dec, int *words_per_block, int *threads, int *post_in_lds,
-                        int *n_levels) {
+                        int *n_levels, int *uniform_deg, int *batch) {
     if (!dec) return fail(LDPC_ERR_ARG, "null decoder");
     if (words_per_block) *words_per_block = dec->P;
     if (threads) *threads = dec->threads;
     if (post_in_lds) *post_in_lds = dec->post_in_lds ? 1 : 0;
     if (n_levels) *n_levels = dec->plan.n_levels;
+    if (uniform_deg) *uniform_deg = dec->uniform_deg;
+    if (batch) *batch = dec->batch;
     return LDPC_OK;
 }
 
@@ -559,6 +606,10 @@ int ldpc_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 }
 int ldpc_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return LDPC_OK;
+}
+int ldpc_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return LDPC_OK;
 }
 int ldpc_device_synchronize(void) {

@@ -83,23 +83,11 @@ __device__ __forceinline__ void magnitudes(int variant, u16x2 min1, u16x2 min2, 
 //   byte and the test a == min1 is made explicit.
 // Returns the per-frame parity-of-positives bits (0x80008000 positions).
 // ---------------------------------------------------------------------------
-template <int D, bool EQ, typename PostPtr>
-__device__ __forceinline__ uint32_t check_d(PostPtr post, const uint32_t *__restrict__ ev,
-                                            uint32_t *__restrict__ mp, int P, int w, bool first,
-                                            int variant, bool nocap, int lparam) {
+template <int D, bool EQ>
+__device__ __forceinline__ uint32_t check_core(const uint32_t (&pw)[D], const uint32_t (&mw)[D], bool first,
+                                               int variant, bool nocap, int lparam, uint32_t (&pn)[D],
+                                               uint32_t (&mn)[D]) {
     constexpr bool KEEP = D <= 12;  // keep |t| and -t live across the two passes
-    uint32_t vi[D];
-#pragma unroll
-    for (int j = 0; j < D; j++) vi[j] = ev[j] * (uint32_t)P + (uint32_t)w;
-    uint32_t pw[D];
-#pragma unroll
-    for (int j = 0; j < D; j++) pw[j] = post[vi[j]];
-    uint32_t mw[D];
-    if (!first) {
-#pragma unroll
-        for (int j = 0; j < D; j++) mw[j] = mp[j];
-    }
-
     u16x2 te[D], to[D], ne[KEEP ? D : 1], no[KEEP ? D : 1], ae[KEEP ? D : 1], ao[KEEP ? D : 1];
     u16x2 m1e = splat(0x7F00u), m2e = m1e, m1o = m1e, m2o = m1e;
     u16x2 se = splat(0u), so = se;
@@ -135,6 +123,9 @@ __device__ __forceinline__ uint32_t check_d(PostPtr post, const uint32_t *__rest
     magnitudes(variant, m1e, m2e, nocap, lparam, k1e, k2e);
     magnitudes(variant, m1o, m2o, nocap, lparam, k1o, k2o);
     const u16x2 Ke = k1e - k2e, Ko = k1o - k2o;
+    // k1 - min(a - min1, k1 - k2) == max(sat_u(k1 + min1 - a), k2): one op
+    // less per half and edge (k1 + min1 <= 0xFE00 cannot wrap)
+    const u16x2 C1e = k1e + m1e, C1o = k1o + m1o;
     se ^= splat(0x8000u);  // top bit now: parity of the others is even -> negate
     so ^= splat(0x8000u);
 
@@ -147,32 +138,128 @@ __device__ __forceinline__ uint32_t check_d(PostPtr post, const uint32_t *__rest
             ye = sub_sat_s(zero, te[j]); yo = sub_sat_s(zero, to[j]);
             be = umin(te[j], ye); bo = umin(to[j], yo);
         }
-        const u16x2 de = be - m1e, dd = bo - m1o;  // >= 0
         u16x2 re, ro;
         if (EQ) {
+            const u16x2 de = be - m1e, dd = bo - m1o;  // >= 0, zero exactly on the minimum
             re = k2e + sub_sat_u(splat(0x0001u), de) * Ke;
             ro = k2o + sub_sat_u(splat(0x0001u), dd) * Ko;
         } else {
-            re = k1e - umin(de, Ke);
-            ro = k1o - umin(dd, Ko);
+            re = umax(sub_sat_u(C1e, be), k2e);
+            ro = umax(sub_sat_u(C1o, bo), k2o);
         }
         const u16x2 se_j = sign_mask(se ^ ye), so_j = sign_mask(so ^ yo);
         const u16x2 me = (re ^ se_j) - se_j, mo = (ro ^ so_j) - so_j;
-        mp[j] = repack(me, mo);
-        post[vi[j]] = repack(add_sat_s(te[j], me), add_sat_s(to[j], mo));
+        mn[j] = repack(me, mo);
+        pn[j] = repack(add_sat_s(te[j], me), add_sat_s(to[j], mo));
     }
     return parity;
 }
 
+template <int D, bool EQ, typename PostPtr>
+__device__ __forceinline__ uint32_t check_d(PostPtr post, const uint32_t *__restrict__ ev,
+                                            uint32_t *__restrict__ mp, int lp, int w, bool first,
+                                            int variant, bool nocap, int lparam) {
+    uint32_t vi[D], pw[D], mw[D], pn[D], mn[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) vi[j] = (ev[j] << lp) + (uint32_t)w;  // P is a power of two
+#pragma unroll
+    for (int j = 0; j < D; j++) pw[j] = post[vi[j]];
+    if (!first) {
+#pragma unroll
+        for (int j = 0; j < D; j++) mw[j] = mp[j];
+    }
+    const uint32_t parity = check_core<D, EQ>(pw, mw, first, variant, nocap, lparam, pn, mn);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        mp[j] = mn[j];
+        post[vi[j]] = pn[j];
+    }
+    return parity;
+}
+
+// Load / store the D message words of one lane (8-byte aligned runs when D is even).
+template <int D>
+__device__ __forceinline__ void load_msgs(const uint32_t *__restrict__ mp, uint32_t (&mw)[D]) {
+    if constexpr (D % 2 == 0) {
+        const uint2 *q = reinterpret_cast<const uint2 *>(mp);
+#pragma unroll
+        for (int h = 0; h < D / 2; h++) {
+            const uint2 x = q[h];
+            mw[2 * h] = x.x;
+            mw[2 * h + 1] = x.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < D; j++) mw[j] = mp[j];
+    }
+}
+template <int D>
+__device__ __forceinline__ void store_msgs(uint32_t *__restrict__ mp, const uint32_t (&mn)[D]) {
+    if constexpr (D % 2 == 0) {
+        uint2 *q = reinterpret_cast<uint2 *>(mp);
+#pragma unroll
+        for (int h = 0; h < D / 2; h++) q[h] = make_uint2(mn[2 * h], mn[2 * h + 1]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < D; j++) mp[j] = mn[j];
+    }
+}
+
+// One level's items of one thread for a code whose checks all have degree D:
+// the check's edges start at c*D, its LDS byte offsets (v * P * 4) come
+// packed two per dword, and the index and message loads of G items are
+// issued together before any of them is computed.  FIRST is a template
+// argument so the first iteration's message-free path is a separate loop.
+template <int D, int G, bool EQ, bool FIRST>
+__device__ __forceinline__ uint32_t level_uniform(const DecodeArgs &a, uint32_t *post, uint32_t *mbase,
+                                                  int c_first, int le, int cstride, int w, int lparam) {
+    constexpr int H = (D + 1) / 2;
+    const int P = a.P;
+    char *const pbase = reinterpret_cast<char *>(post) + 4 * w;
+    uint32_t ov = 0;
+    for (int c0 = c_first; c0 < le; c0 += G * cstride) {
+        uint32_t off[G][H];
+        uint32_t mw[G][D];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int c = c0 + g * cstride;
+            if (c < le) {
+#pragma unroll
+                for (int h = 0; h < H; h++) off[g][h] = a.edge_off16[(size_t)c * H + h];
+                if (!FIRST) load_msgs<D>(mbase + ((size_t)c * P + w) * D, mw[g]);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int c = c0 + g * cstride;
+            if (c < le) {
+                uint32_t *vp[D];
+                uint32_t pw[D], pn[D], mn[D];
+#pragma unroll
+                for (int j = 0; j < D; j++)
+                    vp[j] = reinterpret_cast<uint32_t *>(
+                        pbase + ((j & 1) ? (off[g][j >> 1] >> 16) : (off[g][j >> 1] & 0xFFFFu)));
+#pragma unroll
+                for (int j = 0; j < D; j++) pw[j] = *vp[j];
+                ov |= check_core<D, EQ>(pw, mw[g], FIRST, a.variant, false, lparam, pn, mn);
+                store_msgs<D>(mbase + ((size_t)c * P + w) * D, mn);
+#pragma unroll
+                for (int j = 0; j < D; j++) *vp[j] = pn[j];
+            }
+        }
+    }
+    return ov;
+}
+
 template <bool EQ, int DMAXC, typename PostPtr>
 __device__ __forceinline__ uint32_t check_any(int D, PostPtr post, const uint32_t *__restrict__ ev,
-                                              uint32_t *__restrict__ mp, int P, int w, bool first,
+                                              uint32_t *__restrict__ mp, int lp, int w, bool first,
                                               int variant, bool nocap, int lparam) {
     switch (D) {
 #define LDPC_CASE(d)                                                                        \
     case d:                                                                                 \
         if constexpr (d <= DMAXC)                                                           \
-            return check_d<d, EQ>(post, ev, mp, P, w, first, variant, nocap, lparam);      \
+            return check_d<d, EQ>(post, ev, mp, lp, w, first, variant, nocap, lparam);      \
         else                                                                                \
             return 0;
         LDPC_CASE(1) LDPC_CASE(2) LDPC_CASE(3) LDPC_CASE(4) LDPC_CASE(5) LDPC_CASE(6)
@@ -265,8 +352,8 @@ __device__ void store_block(const DecodeArgs &a, const uint32_t *post, size_t b)
 // ---------------------------------------------------------------------------
 // the decode kernel: one workgroup per block of P words, all iterations
 // ---------------------------------------------------------------------------
-template <bool EQ, bool POST_IN_LDS, int DMAXC>
-__global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_decode(const DecodeArgs a) {
+template <bool EQ, bool POST_IN_LDS, int DMAXC, int UD, int G>
+__global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_decode(const DecodeArgs a) {
     extern __shared__ uint32_t lds_post[];
     __shared__ uint32_t s_ov[MAX_P];
     __shared__ int s_active[MAX_P];
@@ -318,14 +405,22 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_decode(const Decode
         uint32_t ov = 0;
         for (int L = 0; L < a.n_levels; L++) {
             const int lb = a.level_beg[L], le = a.level_beg[L + 1];
-            if (act) {
+            if constexpr (UD > 0) {
+                if (act) {
+                    const int lpar = a.ltable ? (int)s_lrow[UD] : 0;
+                    if (first)
+                        ov |= level_uniform<UD, G, EQ, true>(a, post, mbase, lb + (tid >> lp), le, cstride, w, lpar);
+                    else
+                        ov |= level_uniform<UD, G, EQ, false>(a, post, mbase, lb + (tid >> lp), le, cstride, w, lpar);
+                }
+            } else if (act) {
                 for (int c = lb + (tid >> lp); c < le; c += cstride) {
                     const uint2 ci = a.checks[c];  // x: degree | q1 << 8, y: first edge
                     const int D = (int)(ci.x & 0xFFu);
                     const bool nocap = first && (ci.x & 0x100u) && a.variant == VAR_OMS;
                     const int lparam = a.ltable ? (int)s_lrow[D] : 0;
                     ov |= check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y,
-                                        mbase + (size_t)ci.y * P + (size_t)w * D, P, w, first,
+                                        mbase + (size_t)ci.y * P + (size_t)w * D, lp, w, first,
                                         a.variant, nocap, lparam);
                 }
             }
@@ -411,34 +506,59 @@ __global__ __launch_bounds__(256) void k_deinterleave_hd(const uint32_t *__restr
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <bool EQ, bool LDS, int DMAXC>
+template <bool EQ, bool LDS, int DMAXC, int UD, int G>
 static hipError_t launch_decode_t(const DecodeArgs &a, int threads, hipStream_t s) {
     const size_t lds = LDS ? (size_t)a.N * a.P * 4 : 0;
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_decode<EQ, LDS, DMAXC>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_decode<EQ, LDS, DMAXC, UD, G>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(MAX_LDS_BYTES - 1024));  // static flags live in the rest
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const unsigned nblk = (unsigned)(a.W / a.P);
-    hipLaunchKernelGGL((k_decode<EQ, LDS, DMAXC>), dim3(nblk), dim3(threads), lds, s, a);
+    hipLaunchKernelGGL((k_decode<EQ, LDS, DMAXC, UD, G>), dim3(nblk), dim3(threads), lds, s, a);
     return hipGetLastError();
 }
 
 template <bool EQ, bool LDS>
 static hipError_t launch_dclass(const DecodeArgs &a, int dmax, int threads, hipStream_t s) {
-    if (dmax <= 8) return launch_decode_t<EQ, LDS, 8>(a, threads, s);
-    if (dmax <= 16) return launch_decode_t<EQ, LDS, 16>(a, threads, s);
-    return launch_decode_t<EQ, LDS, 32>(a, threads, s);
+    if (dmax <= 8) return launch_decode_t<EQ, LDS, 8, 0, 1>(a, threads, s);
+    if (dmax <= 16) return launch_decode_t<EQ, LDS, 16, 0, 1>(a, threads, s);
+    return launch_decode_t<EQ, LDS, 32, 0, 1>(a, threads, s);
 }
 
-int max_threads_for_degree(int dmax) { return dmax <= 8 ? 1024 : 256; }
+template <bool EQ, int UD>
+static hipError_t launch_uniform_g(const DecodeArgs &a, int batch, int threads, hipStream_t s) {
+    if (batch >= 4) return launch_decode_t<EQ, true, 8, UD, 4>(a, threads, s);
+    if (batch == 2) return launch_decode_t<EQ, true, 8, UD, 2>(a, threads, s);
+    return launch_decode_t<EQ, true, 8, UD, 1>(a, threads, s);
+}
 
-hipError_t launch_decode(const DecodeArgs &a, int dmax, int threads, bool post_in_lds, hipStream_t s) {
-    if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax)) return hipErrorInvalidValue;
+template <bool EQ>
+static hipError_t launch_uniform(const DecodeArgs &a, int ud, int batch, int threads, hipStream_t s) {
+    switch (ud) {
+    case 6: return launch_uniform_g<EQ, 6>(a, batch, threads, s);
+    case 7: return launch_uniform_g<EQ, 7>(a, batch, threads, s);
+    case 8: return launch_uniform_g<EQ, 8>(a, batch, threads, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+bool uniform_degree_supported(int d) { return d >= 6 && d <= 8; }
+
+int max_threads_for_degree(int dmax, bool uniform) { return uniform ? 512 : (dmax <= 8 ? 1024 : 256); }
+
+hipError_t launch_decode(const DecodeArgs &a, int dmax, int threads, bool post_in_lds, hipStream_t s,
+                         int uniform_deg, int batch) {
+    if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax, uniform_deg > 0)) return hipErrorInvalidValue;
     const bool eq = a.variant == VAR_2NMS;
+    if (uniform_deg > 0) {
+        if (!post_in_lds || !a.edge_off16 || uniform_deg != dmax) return hipErrorInvalidValue;
+        return eq ? launch_uniform<true>(a, uniform_deg, batch, threads, s)
+                  : launch_uniform<false>(a, uniform_deg, batch, threads, s);
+    }
     if (post_in_lds) return eq ? launch_dclass<true, true>(a, dmax, threads, s) : launch_dclass<false, true>(a, dmax, threads, s);
     return eq ? launch_dclass<true, false>(a, dmax, threads, s) : launch_dclass<false, false>(a, dmax, threads, s);
 }

@@ -19,6 +19,8 @@ struct DecodeArgs {
     // code plan (level-sorted)
     const uint2 *checks;       // [C] x = degree | q1 << 8, y = first edge (level order)
     const uint32_t *edge_var;  // [E] variable index per level-order edge
+    const uint32_t *edge_off16;  // uniform-degree codes: per check (D+1)/2 dwords of
+                                 // LDS byte offsets v*P*4 packed two per dword, or null
     const int *level_beg;      // [n_levels + 1]
     int n_levels, N, C, E;
     // blocking
@@ -41,8 +43,12 @@ struct DecodeArgs {
     int tits, tdeg, iter_base;
 };
 
-hipError_t launch_decode(const DecodeArgs &a, int dmax, int threads, bool post_in_lds, hipStream_t s);
-int max_threads_for_degree(int dmax);
+// uniform_deg > 0 selects the single-degree kernel (needs edge_off16 and the
+// posterior in LDS); batch = items whose loads are issued together (1, 2, 4).
+hipError_t launch_decode(const DecodeArgs &a, int dmax, int threads, bool post_in_lds, hipStream_t s,
+                         int uniform_deg, int batch);
+bool uniform_degree_supported(int d);
+int max_threads_for_degree(int dmax, bool uniform);
 hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipStream_t s);
 hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s);
 

@@ -113,6 +113,9 @@ typedef struct {
     int threads;          /* workgroup size: 0 = auto                         */
     int early_term;       /* -1 = variant default (2NMS: on), 0 off, 1 on      */
     int post_in_lds;      /* -1 = auto (posterior block fits LDS), 0, 1        */
+    int uniform;          /* single-degree kernel: -1 = auto, 0 = off, 1 = on  */
+    int batch;            /* checks per lane whose loads are issued together:
+                             0 = auto, 1, 2 or 4 (single-degree kernel)        */
 } ldpc_tuning;
 
 /* CGPUDecoder(nb_frames, n, k, m) + derived ctor: device buffers and the H
@@ -137,10 +140,11 @@ int ldpc_decoder_decode_device(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d
 
 /* The decoder's own HIP stream (hipStream_t) used when no stream is given. */
 void *ldpc_decoder_stream(ldpc_decoder *dec);
-/* Words per workgroup block, workgroup size, posterior-in-LDS flag and
- * dependency levels chosen at create time. Any pointer may be NULL. */
+/* Words per workgroup block, workgroup size, posterior-in-LDS flag,
+ * dependency levels, single-degree kernel degree (0 = generic kernel) and
+ * load batch chosen at create time. Any pointer may be NULL. */
 int ldpc_decoder_config(const ldpc_decoder *dec, int *words_per_block, int *threads,
-                        int *post_in_lds, int *n_levels);
+                        int *post_in_lds, int *n_levels, int *uniform_deg, int *batch);
 
 /* Learned table for LDPC_LOMS / LDPC_LNMS: int8 [n_iters][n_degs], row =
  * absolute iteration (last row repeats), column = check degree (last column
@@ -179,6 +183,8 @@ void *ldpc_host_alloc_pinned(size_t bytes);
 void ldpc_host_free_pinned(void *p);
 int ldpc_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int ldpc_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy (a plain copy of known size: HBM counter calibration). */
+int ldpc_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int ldpc_device_synchronize(void);
 /* Number of visible gfx950 devices (0 without a GPU). */
 int ldpc_device_count(void);

@@ -176,12 +176,35 @@ def test_odd_length_frames_byte_path(gpu_available):
     run_lockstep(code, W, L.OMS, llr, 5)
 
 
-@pytest.mark.parametrize("P,T", [(1, 64), (2, 128), (8, 256), (16, 1024), (4, 512)])
-def test_tunings(P, T, gpu_available):
-    code = code36() if P <= 8 else L.code_3_6(1200, seed=7)
+@pytest.mark.parametrize("P,T,uniform", [(1, 64, 1), (2, 128, 1), (8, 256, 0), (4, 256, 1), (16, 1024, 0), (16, 512, 1),
+                                         (4, 512, 0), (4, 256, 0)])
+def test_tunings(P, T, uniform, gpu_available):
+    code = code36() if P <= 8 else L.code_3_6(1000, seed=7)
     W = 32
     llr = channel_llr(code.n_vars, 4 * W, seed=P * T)
-    run_lockstep(code, W, L.OMS, llr, 5, words_per_block=P, threads=T)
+    dec, _ = run_lockstep(code, W, L.OMS, llr, 5, words_per_block=P, threads=T, uniform=uniform)
+    assert dec.config()["uniform_deg"] == (6 if uniform else 0)
+
+
+@pytest.mark.parametrize("batch", [1, 2, 4])
+@pytest.mark.parametrize("variant", [L.OMS, L.NMS2, L.LOMS], ids=lambda v: L.VARIANT_NAMES[v])
+def test_single_degree_kernel_batches(batch, variant, gpu_available):
+    code = code36()
+    W = 32
+    llr = channel_llr(4000, 4 * W, seed=batch + 10 * variant)
+    dec, _ = run_lockstep(code, W, variant, llr, 6, uniform=1, batch=batch, early_term=1 if variant == L.NMS2 else 0)
+    assert dec.config()["uniform_deg"] == 6 and dec.config()["batch"] == batch
+
+
+@pytest.mark.parametrize("d", [7, 8])
+def test_single_degree_kernel_other_degrees(d, gpu_available):
+    n = {7: 1400, 8: 1600}[d]
+    code = L.Code.regular(n, 4, d, seed=d).reorder_by_colour() if d == 8 else L.Code.regular(n, 3, d, seed=d).reorder_by_colour()
+    W = 16
+    for batch in (1, 4):
+        llr = channel_llr(n, 4 * W, seed=d * batch)
+        dec, _ = run_lockstep(code, W, L.OMS, llr, 4, uniform=1, batch=batch)
+        assert dec.config()["uniform_deg"] == d
 
 
 def test_global_posterior_path(gpu_available):
