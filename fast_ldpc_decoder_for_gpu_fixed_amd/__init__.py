@@ -197,6 +197,13 @@ class CGPUDecoder:
             raise LdpcError("table must be 2-D [iteration][degree]")
         check(load().ldpc_decoder_set_table(self._h, ptr(t, C.c_int8), t.shape[0], t.shape[1]), "set_table")
 
+    def count_errors(self, d_hard: int, frames: int, n_info: int, d_counters: int, d_frame_bits: int = 0,
+                     d_frame_ok: int = 0, accumulate: bool = False, stream: int = 0) -> None:
+        """Device error/syndrome counters (adds to three uint64 at d_counters)."""
+        v = lambda p: C.c_void_p(p) if p else None
+        check(load().ldpc_decoder_count_errors(self._h, v(d_hard), frames, n_info, v(d_counters), v(d_frame_bits),
+                                               v(d_frame_ok), int(accumulate), v(stream)), "count_errors")
+
     # ---- state API ----
     def load(self, var_nodes) -> None:
         a = self._in(var_nodes)
@@ -317,6 +324,17 @@ class DeviceBuffer:
         if p and _lib._lib is not None:
             _lib._lib.ldpc_device_free(C.c_void_p(p))
             self.ptr = None
+
+
+def channel_sigma(snr_db: float, rate: float, es_n0: bool = False) -> float:
+    return float(load().ldpc_channel_sigma(snr_db, rate, int(es_n0)))
+
+
+def channel_generate(d_out: int, n_vars: int, frames: int, seed: int, snr_idx: int, sigma: float,
+                     quant: int = 0, layout: int = 0, first_frame: int = 0, stream: int = 0) -> None:
+    """Device AWGN channel into d_out (layout 0: frame-major int8, 1: device words)."""
+    check(load().ldpc_channel_generate(C.c_void_p(d_out), layout, n_vars, frames, first_frame, seed, snr_idx,
+                                       sigma, quant, C.c_void_p(stream) if stream else None), "channel_generate")
 
 
 def device_count() -> int:

@@ -50,6 +50,11 @@ SIGNATURES = {
     "ldpc_decoder_get_stop_info": (C.c_int, [C.c_void_p, _u8_p, _c_int_p]),
     "ldpc_interleave_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]),
     "ldpc_deinterleave_hd_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]),
+    "ldpc_channel_generate": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_size_t, C.c_uint64, C.c_uint64,
+                                        C.c_uint32, C.c_double, C.c_int, C.c_void_p]),
+    "ldpc_channel_sigma": (C.c_double, [C.c_double, C.c_double, C.c_int]),
+    "ldpc_decoder_count_errors": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_int, C.c_void_p]),
     "ldpc_device_alloc": (C.c_void_p, [C.c_size_t]),
     "ldpc_device_free": (None, [C.c_void_p]),
     "ldpc_host_alloc_pinned": (C.c_void_p, [C.c_size_t]),

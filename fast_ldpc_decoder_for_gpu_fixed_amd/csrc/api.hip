@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -576,6 +577,42 @@ int ldpc_interleave_device(const int8_t *d_in, uint32_t *d_words, int n_vars, si
 int ldpc_deinterleave_hd_device(const uint32_t *d_words, uint8_t *d_out, int n_vars, size_t W, void *stream) {
     if (!d_words || !d_out || n_vars <= 0 || W == 0 || W > 0x7FFFFFFF) return fail(LDPC_ERR_ARG, "bad arguments");
     HIP_TRY(ldpc::launch_deinterleave_hd(d_words, d_out, n_vars, (int)W, (hipStream_t)stream));
+    return LDPC_OK;
+}
+
+// ---- channel and error counting ------------------------------------------------
+int ldpc_channel_generate(void *d_out, int layout, int n_vars, size_t frames, uint64_t first_frame, uint64_t seed,
+                          uint32_t snr_idx, double sigma, int quant, void *stream) {
+    if (!d_out || n_vars <= 0 || frames == 0 || (layout != 0 && layout != 1) || (quant != 0 && quant != 1) ||
+        !(sigma > 0.0))
+        return fail(LDPC_ERR_ARG, "ldpc_channel_generate: bad arguments");
+    if (layout == 1 && frames % 4) return fail(LDPC_ERR_ARG, "device-word layout needs frames % 4 == 0");
+    HIP_TRY(ldpc::launch_channel(d_out, layout, n_vars, frames, first_frame, seed, snr_idx, sigma, quant,
+                                 (hipStream_t)stream));
+    return LDPC_OK;
+}
+
+double ldpc_channel_sigma(double snr_db, double rate, int es_n0) {
+    double ebn0 = es_n0 ? snr_db - 10.0 * std::log10(2.0 * rate) : snr_db;
+    const double interm = -0.1 * (ebn0 + 10.0 * std::log10(rate));
+    return std::sqrt(std::pow(10.0, interm) / 2.0);
+}
+
+int ldpc_decoder_count_errors(ldpc_decoder *dec, const uint8_t *d_hard, size_t frames, int n_info,
+                              uint64_t *d_counters, uint32_t *d_frame_bit_errors, uint8_t *d_frame_syndrome_ok,
+                              int accumulate, void *stream) {
+    if (!dec || !d_hard || !d_counters || frames == 0 || n_info < 0 || n_info > dec->code.N)
+        return fail(LDPC_ERR_ARG, "ldpc_decoder_count_errors: bad arguments");
+    if ((size_t)dec->code.N > ldpc::MAX_LDS_BYTES - 1024)
+        return fail(LDPC_ERR_ARG, "frame longer than the LDS staging buffer");
+    static bool attr = false;
+    if (!attr) {
+        HIP_TRY(ldpc::set_count_lds_limit());
+        attr = true;
+    }
+    HIP_TRY(ldpc::launch_count(d_hard, dec->code.N, frames, n_info, dec->d_checks, dec->d_edge_var, dec->code.C,
+                               reinterpret_cast<unsigned long long *>(d_counters), d_frame_bit_errors,
+                               d_frame_syndrome_ok, accumulate == 0, (hipStream_t)stream));
     return LDPC_OK;
 }
 

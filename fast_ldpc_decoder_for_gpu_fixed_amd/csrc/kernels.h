@@ -52,4 +52,11 @@ int max_threads_for_degree(int dmax, bool uniform);
 hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipStream_t s);
 hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s);
 
+hipError_t launch_channel(void *out, int layout, int N, size_t frames, uint64_t f0, uint64_t seed, uint32_t snr_idx,
+                          double sigma, int quant, hipStream_t s);
+hipError_t set_count_lds_limit();
+hipError_t launch_count(const uint8_t *hard, int N, size_t frames, int n_info, const uint2 *checks,
+                        const uint32_t *edge_var, int C, unsigned long long *counters, uint32_t *per_frame_bits,
+                        uint8_t *per_frame_ok, bool zero_first, hipStream_t s);
+
 }  // namespace ldpc

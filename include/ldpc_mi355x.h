@@ -173,6 +173,33 @@ int ldpc_interleave_device(const int8_t *d_in, uint32_t *d_words, int n_vars,
 int ldpc_deinterleave_hd_device(const uint32_t *d_words, uint8_t *d_out, int n_vars,
                                 size_t nb_frames_words, void *stream);
 
+/* ---- device AWGN channel (replaces CChanel_AWGN_SIMD::configure/generate,
+ * src/awgn_channel/CChanel_AWGN_SIMD.cu:63-112: cuRAND + Box-Muller + a D2H copy
+ * per chunk).  All-zero codeword, BPSK bit 0 -> -1, y = -1 + sigma*z; sample
+ * s = frame*N + var is a pure function of (seed, snr_idx, s): Philox4x32-10
+ * counter (s_lo, s_hi, snr_idx, 0x4C445043), key (seed_lo, seed_hi), two
+ * 53-bit uniforms, Box-Muller cosine branch, in double precision.
+ * quant 0: sat127(round(16*y/sigma^2))   (8 * the LLR 2y/sigma^2)
+ * quant 1: (int8)trunc(clamp(8*y, +-31)) (CChanel_AWGN_SIMD.cu:23-24)
+ * layout 0: frame-major int8 [frames][N] (the decode input);
+ * layout 1: device words [N][frames/4] (OMS_v2 input / interleaved state). */
+int ldpc_channel_generate(void *d_out, int layout, int n_vars, size_t frames, uint64_t first_frame,
+                          uint64_t seed, uint32_t snr_idx, double sigma, int quant, void *stream);
+/* sigma of CChanel_AWGN_SIMD::configure (:63-73); es_n0 != 0 reads `snr_db`
+ * as Es/N0. */
+double ldpc_channel_sigma(double snr_db, double rate, int es_n0);
+
+/* ---- device error counting (replaces CErrorAnalyzer::generate,
+ * src/ber_analyzer/CErrorAnalyzer.cpp:141-157, a host scan) -----------------
+ * d_hard: frame-major decisions [frames][N] (all-zero codeword, so a 1 is an
+ * error).  Adds to d_counters[0..2] (unsigned 64-bit, device memory): bit
+ * errors in positions < n_info, frames with a bit error, frames whose
+ * decisions violate a parity check of `dec`'s code.  accumulate = 0 zeroes
+ * the counters first.  Per-frame outputs may be NULL. Asynchronous. */
+int ldpc_decoder_count_errors(ldpc_decoder *dec, const uint8_t *d_hard, size_t frames, int n_info,
+                              uint64_t *d_counters, uint32_t *d_frame_bit_errors,
+                              uint8_t *d_frame_syndrome_ok, int accumulate, void *stream);
+
 /* ---- device memory helpers (plumbing for callers without a HIP runtime
  * binding; replaces CUDA_MALLOC_DEVICE/CUDA_MALLOC_HOST,
  * src/custom_api/custom_cuda.h:24-40, and CTrame's pinned buffers,

@@ -380,7 +380,8 @@ int8_t orc_llr(uint64_t seed, uint32_t snr_idx, uint64_t s, double sigma, int qu
         float q = fminf(fmaxf(8.0f * v, -31.0f), 31.0f);
         return (int8_t)q;
     }
-    double q = round(16.0 * y / (sigma * sigma)); /* 8 * (2y / sigma^2) */
+    const double scale = 16.0 / (sigma * sigma); /* 8 * (2y / sigma^2) */
+    double q = round(scale * y);
     if (q > 127.0) q = 127.0;
     if (q < -127.0) q = -127.0;
     return (int8_t)q;
