@@ -46,9 +46,11 @@ def test_channel_statistics_full_batch(gpu_available):
     L.channel_generate(d.ptr, N, frames, 1, 0, sig)
     L.synchronize()
     x = d.download(np.empty(frames * N, np.int8))
-    p = 0.5 * math.erfc((1 / sig) / math.sqrt(2))
-    frac = float((x > 0).mean())
-    assert abs(frac - p) < 6 * math.sqrt(p * (1 - p) / x.size)
+    # round(16y/sigma^2) >= 1  <=>  y >= sigma^2/32;  >= 0  <=>  y > -sigma^2/32
+    Q = lambda t: 0.5 * math.erfc(t / math.sqrt(2))
+    for frac, p in ((float((x > 0).mean()), Q((1 + sig ** 2 / 32) / sig)),
+                    (float((x >= 0).mean()), Q((1 - sig ** 2 / 32) / sig))):
+        assert abs(frac - p) < 6 * math.sqrt(p * (1 - p) / x.size)
 
 
 def test_device_counters_match_oracle(gpu_available):
