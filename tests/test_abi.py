@@ -58,3 +58,33 @@ def test_grouped_form_matches_table_form():
     assert g.q1_split == 6 == t.q1_split
     for a, b in zip(g.export(), t.export()):
         np.testing.assert_array_equal(a, b)
+
+
+def test_cpp_class_header_compiles_and_links(tmp_path):
+    """include/CGPUDecoder.hpp (the reference's class shape) builds against the library."""
+    import subprocess
+    root = HEADER.parents[1]
+    src = tmp_path / "use_header.cpp"
+    src.write_text(r'''
+#include "CGPUDecoder.hpp"
+#include <cstdio>
+int main() {
+    ldpc_code* code = ldpc_code_regular(4000, 3, 6, 1);
+    if (!code) return 1;
+    int n, k, m;
+    ldpc_code_info(code, &n, &k, &m, nullptr, nullptr, nullptr);
+    if (ldpc_device_count() == 0) { std::printf("no device\n"); ldpc_code_destroy(code); return 0; }
+    CGPUDecoder* d = new CGPU_Decoder_OMS_SIMD(1024, n, k, m, code);
+    d->initialize();
+    delete d;
+    ldpc_code_destroy(code);
+    return 0;
+}
+''')
+    pkg = root / "fast_ldpc_decoder_for_gpu_fixed_amd"
+    exe = tmp_path / "use_header"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", f"-I{root / 'include'}", str(src), "-o", str(exe),
+                        f"-L{pkg}", "-lldpc_mi355x", f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert run.returncode == 0, run.stdout + run.stderr
