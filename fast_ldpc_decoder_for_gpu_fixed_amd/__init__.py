@@ -19,7 +19,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import LdpcError, Tuning, check, load, ptr
+from ._lib import LdpcError, LearnConfig, SimStats, Tuning, check, load, ptr
 
 MS, OMS, OMS_V2, NMS, NMS2, LOMS, LNMS = range(7)
 VARIANT_NAMES = {MS: "MS", OMS: "OMS", OMS_V2: "OMS_v2", NMS: "NMS", NMS2: "2NMS",
@@ -204,6 +204,18 @@ class CGPUDecoder:
         check(load().ldpc_decoder_count_errors(self._h, v(d_hard), frames, n_info, v(d_counters), v(d_frame_bits),
                                                v(d_frame_ok), int(accumulate), v(stream)), "count_errors")
 
+    def simulate(self, ebn0_db: float, n_frames: int, iterations: int, seed: int = 1, snr_idx: int = 0,
+                 quant: int = 0) -> dict:
+        """One Eb/N0 point on the device: channel -> decode -> count (frames
+        k*4W.. of the stream (seed, snr_idx), identical for every decoder)."""
+        st = SimStats()
+        check(load().ldpc_simulate(self._h, ebn0_db, quant, seed, snr_idx, n_frames, iterations, C.byref(st)),
+              "simulate")
+        return {"frames": st.frames, "bit_errors": st.bit_errors, "frame_errors": st.frame_errors,
+                "syndrome_failures": st.syndrome_failures, "avg_iterations": st.avg_iterations,
+                "decode_ms": st.decode_ms, "fer": st.frame_errors / st.frames,
+                "ber": st.bit_errors / (st.frames * (self.n - self.k))}
+
     # ---- state API ----
     def load(self, var_nodes) -> None:
         a = self._in(var_nodes)
@@ -335,6 +347,64 @@ def channel_generate(d_out: int, n_vars: int, frames: int, seed: int, snr_idx: i
     """Device AWGN channel into d_out (layout 0: frame-major int8, 1: device words)."""
     check(load().ldpc_channel_generate(C.c_void_p(d_out), layout, n_vars, frames, first_frame, seed, snr_idx,
                                        sigma, quant, C.c_void_p(stream) if stream else None), "channel_generate")
+
+
+def channel_generate_range(d_out: int, n_vars: int, frames: int, seed: int, lo_db: float, hi_db: float,
+                           rate: float, quant: int = 0, first_frame: int = 0, snr_idx: int = 0,
+                           stream: int = 0) -> None:
+    check(load().ldpc_channel_generate_range(C.c_void_p(d_out), n_vars, frames, first_frame, seed, snr_idx, lo_db,
+                                             hi_db, rate, quant, C.c_void_p(stream) if stream else None),
+          "channel_generate_range")
+
+
+LEARN_OFFSET, LEARN_SCALE = 0, 1
+
+
+class Trainer:
+    """Float shadow decoder on the GPU (forward + backward HIP kernels) and
+    Adam, for per-iteration / per-degree offset or scale tables."""
+
+    def __init__(self, code: Code, mode: int, iterations: int, n_degs: int, batch_frames: int):
+        h = load().ldpc_trainer_create(code.handle, mode, iterations, n_degs, batch_frames)
+        if not h:
+            raise LdpcError(f"ldpc_trainer_create failed: {_lib.last_error()}")
+        self._h = C.c_void_p(h)
+        self.code, self.mode, self.T, self.dcols, self.frames = code, mode, iterations, n_degs, batch_frames
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.ldpc_trainer_destroy(h)
+            self._h = None
+
+    def loss_grad(self, params, d_llr: int = 0, want_grad: bool = True, d_post_out: int = 0):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.shape == (self.T, self.dcols)
+        loss = C.c_float()
+        grad = np.zeros_like(p) if want_grad else None
+        check(load().ldpc_trainer_loss_grad(self._h, ptr(p, C.c_float), C.c_void_p(d_llr) if d_llr else None,
+                                            C.byref(loss), ptr(grad, C.c_float) if want_grad else None,
+                                            C.c_void_p(d_post_out) if d_post_out else None), "trainer_loss_grad")
+        return float(loss.value), grad
+
+    def fit(self, params, steps: int, lr: float = 1e-2, ebn0=(1.5, 2.5), seed: int = 1, first_frame: int = 0,
+            quant: int = 0, pmin: float = 0.0, pmax: float = 8.0):
+        p = np.ascontiguousarray(params, dtype=np.float32).copy()
+        hist = np.zeros(steps, np.float32)
+        cfg = LearnConfig(steps, lr, 0.9, 0.999, 1e-8, ebn0[0], ebn0[1], 0.0, seed, first_frame, quant, pmin, pmax)
+        check(load().ldpc_trainer_fit(self._h, C.byref(cfg), ptr(p, C.c_float), ptr(hist, C.c_float)),
+              "trainer_fit")
+        return p, hist
+
+
+def quantize_offsets(beta: np.ndarray) -> np.ndarray:
+    """Float offsets (LSB) -> int8 steps for LDPC_LOMS."""
+    return np.clip(np.rint(beta), 0, 31).astype(np.int8)
+
+
+def quantize_scales(alpha: np.ndarray) -> np.ndarray:
+    """Float scales -> numerators over 8 for LDPC_LNMS."""
+    return np.clip(np.rint(8 * alpha), 0, 8).astype(np.int8)
 
 
 def device_count() -> int:

@@ -21,6 +21,20 @@ class Tuning(C.Structure):
                 ("batch", C.c_int)]
 
 
+class SimStats(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("bit_errors", C.c_uint64), ("frame_errors", C.c_uint64),
+                ("syndrome_failures", C.c_uint64), ("avg_iterations", C.c_double), ("decode_ms", C.c_double)]
+
+
+class LearnConfig(C.Structure):
+    _fields_ = [("steps", C.c_int), ("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double),
+                ("eps", C.c_double), ("ebn0_lo", C.c_double), ("ebn0_hi", C.c_double), ("rate", C.c_double),
+                ("seed", C.c_uint64), ("first_frame", C.c_uint64), ("quant", C.c_int), ("pmin", C.c_double),
+                ("pmax", C.c_double)]
+
+
+_f_p = C.POINTER(C.c_float)
+
 # name -> (restype, argtypes); mirrors include/ldpc_mi355x.h one to one
 SIGNATURES = {
     "ldpc_last_error": (C.c_char_p, []),
@@ -55,6 +69,14 @@ SIGNATURES = {
     "ldpc_channel_sigma": (C.c_double, [C.c_double, C.c_double, C.c_int]),
     "ldpc_decoder_count_errors": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_int, C.c_void_p]),
+    "ldpc_channel_generate_range": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_uint64, C.c_uint64, C.c_uint32,
+                                              C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p]),
+    "ldpc_simulate": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int,
+                                C.POINTER(SimStats)]),
+    "ldpc_trainer_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_size_t]),
+    "ldpc_trainer_destroy": (None, [C.c_void_p]),
+    "ldpc_trainer_loss_grad": (C.c_int, [C.c_void_p, _f_p, C.c_void_p, _f_p, _f_p, C.c_void_p]),
+    "ldpc_trainer_fit": (C.c_int, [C.c_void_p, C.POINTER(LearnConfig), _f_p, _f_p]),
     "ldpc_device_alloc": (C.c_void_p, [C.c_size_t]),
     "ldpc_device_free": (None, [C.c_void_p]),
     "ldpc_host_alloc_pinned": (C.c_void_p, [C.c_size_t]),

@@ -59,6 +59,10 @@ hipError_t dalloc(T **p, size_t count) {
 
 }  // namespace
 
+namespace ldpc {
+void set_last_error(const std::string &m) { g_err = m; }
+}  // namespace ldpc
+
 struct ldpc_decoder {
     ldpc_code code;
     Plan plan;
@@ -369,7 +373,8 @@ ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
     ALLOC(d_iters, W);
     ALLOC(d_edge_off16, off16.size());
 #undef ALLOC
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    // a blocking stream: work the caller enqueues on the null stream stays ordered with it
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamDefault);
     if (e == hipSuccess) e = hipMemcpy(d->d_checks, ck.data(), ck.size() * sizeof(uint2), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d->d_edge_var, d->plan.edge_var.data(), (size_t)c.E * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && !off16.empty())
@@ -613,6 +618,63 @@ int ldpc_decoder_count_errors(ldpc_decoder *dec, const uint8_t *d_hard, size_t f
     HIP_TRY(ldpc::launch_count(d_hard, dec->code.N, frames, n_info, dec->d_checks, dec->d_edge_var, dec->code.C,
                                reinterpret_cast<unsigned long long *>(d_counters), d_frame_bit_errors,
                                d_frame_syndrome_ok, accumulate == 0, (hipStream_t)stream));
+    return LDPC_OK;
+}
+
+// ---- simulation loop (src/main.cpp:237-335 for one Eb/N0 point) -----------------
+int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, uint32_t snr_idx, uint64_t n_frames,
+                  int iterations, ldpc_sim_stats *out) {
+    if (!dec || !out || n_frames == 0) return fail(LDPC_ERR_ARG, "ldpc_simulate: bad arguments");
+    int rc = check_iters(iterations);
+    if (rc) return rc;
+    const ldpc_code &c = dec->code;
+    const double rate = (double)(c.N - c.C) / (double)c.N;
+    const double sigma = ldpc_channel_sigma(ebn0_db, rate, 0);
+    const size_t batch = 4 * dec->W;
+    const uint64_t batches = (n_frames + batch - 1) / batch;
+    uint64_t *d_cnt = nullptr;
+    HIP_TRY(hipMalloc((void **)&d_cnt, 3 * sizeof(uint64_t)));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    std::vector<int> iters(dec->W);
+    double ms = 0.0, it_sum = 0.0;
+    hipStream_t s = dec->stream;
+    for (uint64_t k = 0; k < batches && rc == LDPC_OK; k++) {
+        rc = ldpc_channel_generate(dec->d_in, dec->variant == LDPC_OMS_V2 ? 1 : 0, c.N, batch, k * batch, seed,
+                                   snr_idx, sigma, quant, s);
+        if (rc) break;
+        hipEventRecord(e0, s);
+        rc = ldpc_decoder_decode_device(dec, dec->d_in, dec->d_out, iterations, s);
+        hipEventRecord(e1, s);
+        if (rc) break;
+        rc = ldpc_decoder_count_errors(dec, dec->d_out, batch, c.N - c.C, d_cnt, nullptr, nullptr, k > 0, s);
+        if (rc) break;
+        if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(LDPC_ERR_HIP, "event sync"); break; }
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        ms += t;
+        if (dec->early_term) {
+            if (hipMemcpyAsync(iters.data(), dec->d_iters, dec->W * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) { rc = fail(LDPC_ERR_HIP, "iteration counts"); break; }
+            for (int x : iters) it_sum += 4.0 * x;
+        } else {
+            it_sum += (double)batch * iterations;
+        }
+    }
+    uint64_t cnt[3] = {0, 0, 0};
+    if (rc == LDPC_OK && hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(LDPC_ERR_HIP, "counter copy");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(d_cnt);
+    if (rc) return rc;
+    out->frames = batches * batch;
+    out->bit_errors = cnt[0];
+    out->frame_errors = cnt[1];
+    out->syndrome_failures = cnt[2];
+    out->avg_iterations = it_sum / (double)out->frames;
+    out->decode_ms = ms;
     return LDPC_OK;
 }
 

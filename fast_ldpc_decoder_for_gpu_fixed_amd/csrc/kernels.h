@@ -52,8 +52,31 @@ int max_threads_for_degree(int dmax, bool uniform);
 hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipStream_t s);
 hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s);
 
+// float shadow decoder (learned tables)
+struct ShadowArgs {
+    const uint2 *checks;
+    const uint32_t *edge_var;
+    const int *level_beg;
+    int n_levels, N, C, E;
+    int FB, lfb;            // frames per workgroup (power of two)
+    size_t frames;          // multiple of FB
+    const int8_t *llr;      // frame-major [frames][N]
+    float *msg;             // [frames/FB][E][FB], per check [frame][j]; gradients in reverse
+    float *rstore;          // [T][frames/FB][E*FB] pre-clamp v2c values
+    float *pend;            // [T][frames/FB][N*FB] posterior after each iteration
+    const float *params;    // [T][dcols]
+    float *grad;            // [T][dcols], accumulated with atomics
+    float *loss;            // one float, accumulated with atomics
+    float *post_out;        // optional [frames][N] posterior after T iterations
+    int T, dcols, mode;     // mode 0 offset, 1 scale
+};
+hipError_t launch_shadow(const ShadowArgs &a, bool backward, hipStream_t s);
+
 hipError_t launch_channel(void *out, int layout, int N, size_t frames, uint64_t f0, uint64_t seed, uint32_t snr_idx,
                           double sigma, int quant, hipStream_t s);
+// per-frame Eb/N0 uniform in [lo, hi] dB (frame-major output)
+hipError_t launch_channel_range(int8_t *out, int N, size_t frames, uint64_t f0, uint64_t seed, uint32_t snr_idx,
+                                double lo_db, double hi_db, double rate, int quant, hipStream_t s);
 hipError_t set_count_lds_limit();
 hipError_t launch_count(const uint8_t *hard, int N, size_t frames, int n_info, const uint2 *checks,
                         const uint32_t *edge_var, int C, unsigned long long *counters, uint32_t *per_frame_bits,

@@ -76,6 +76,32 @@ __global__ __launch_bounds__(256) void k_channel_frames(int8_t *__restrict__ out
     }
 }
 
+// Eb/N0 of frame f for the range channel: lo + (hi - lo) * u, u from its own
+// Philox block (counter (f_lo, f_hi, snr_idx, 0x534E5252)).
+__device__ __forceinline__ double frame_sigma(uint64_t seed, uint32_t snr_idx, uint64_t f, double lo, double hi,
+                                              double rate) {
+    uint32_t c[4] = {(uint32_t)f, (uint32_t)(f >> 32), snr_idx, 0x534E5252u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double ebn0 = lo + (hi - lo) * ((double)c[0] * 0x1.0p-32);
+    return sqrt(0.5 / (rate * pow(10.0, ebn0 / 10.0)));
+}
+
+__global__ __launch_bounds__(256) void k_channel_range(int8_t *__restrict__ out, int N, size_t frames, uint64_t f0,
+                                                       uint64_t seed, uint32_t snr_idx, double lo, double hi,
+                                                       double rate, int quant) {
+    const size_t n4 = (size_t)(N + 3) / 4;
+    const size_t total = frames * n4;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t f = i / n4;
+        const int v0 = (int)(i % n4) * 4;
+        const double sigma = frame_sigma(seed, snr_idx, f0 + f, lo, hi, rate);
+        const double scale = 16.0 / (sigma * sigma);
+        const uint64_t s0 = (f0 + f) * (uint64_t)N;
+        for (int r = 0; r < 4 && v0 + r < N; r++)
+            out[f * (size_t)N + v0 + r] = channel_llr(seed, snr_idx, s0 + v0 + r, sigma, scale, quant);
+    }
+}
+
 // device layout: word [v][w] holds frames 4w..4w+3 of variable v
 __global__ __launch_bounds__(256) void k_channel_words(uint32_t *__restrict__ out, int N, size_t W, uint64_t f0,
                                                        uint64_t seed, uint32_t snr_idx, double sigma, double scale,
@@ -139,6 +165,15 @@ __global__ __launch_bounds__(256) void k_count(const uint8_t *__restrict__ hard,
         }
         __syncthreads();
     }
+}
+
+hipError_t launch_channel_range(int8_t *out, int N, size_t frames, uint64_t f0, uint64_t seed, uint32_t snr_idx,
+                                double lo_db, double hi_db, double rate, int quant, hipStream_t s) {
+    const size_t items = frames * (size_t)((N + 3) / 4);
+    const unsigned grid = (unsigned)std::min<size_t>((items + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_channel_range, dim3(grid), dim3(256), 0, s, out, N, frames, f0, seed, snr_idx, lo_db, hi_db,
+                       rate, quant);
+    return hipGetLastError();
 }
 
 hipError_t set_count_lds_limit() {

@@ -200,6 +200,59 @@ int ldpc_decoder_count_errors(ldpc_decoder *dec, const uint8_t *d_hard, size_t f
                               uint64_t *d_counters, uint32_t *d_frame_bit_errors,
                               uint8_t *d_frame_syndrome_ok, int accumulate, void *stream);
 
+/* Training frames: per-frame Eb/N0 = lo + (hi-lo)*u(f) dB, u from the Philox
+ * block (f_lo, f_hi, snr_idx, 0x534E5252); sigma = sqrt(0.5/(rate*10^(EbN0/10)));
+ * frame-major int8, otherwise as ldpc_channel_generate. */
+int ldpc_channel_generate_range(void *d_out, int n_vars, size_t frames, uint64_t first_frame,
+                                uint64_t seed, uint32_t snr_idx, double ebn0_lo, double ebn0_hi,
+                                double rate, int quant, void *stream);
+
+/* ---- simulation of one Eb/N0 point (the loop body of src/main.cpp:237-335:
+ * channel -> decode -> error count, repeated over batches; device only) --- */
+typedef struct {
+    uint64_t frames;             /* frames decoded (multiple of 4W)            */
+    uint64_t bit_errors;         /* in the first N - #checks positions          */
+    uint64_t frame_errors;
+    uint64_t syndrome_failures;  /* frames whose decisions fail a parity check */
+    double avg_iterations;       /* per frame (early stop counts actual ones)  */
+    double decode_ms;            /* hipEvent time of the decode kernels only   */
+} ldpc_sim_stats;
+/* Frames i*4W .. of the stream (seed, snr_idx), so two decoders given the same
+ * arguments see identical frames. */
+int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, uint32_t snr_idx,
+                  uint64_t n_frames, int iterations, ldpc_sim_stats *out);
+
+/* ---- learned tables: float shadow decoder trained on the GPU ----------------
+ * (no counterpart in the reference).  The shadow runs the integer decoder's
+ * layered schedule in float LSB units with a per-iteration, per-check-degree
+ * offset (mode OFFSET: mag = clamp(min - beta, 0, 31)) or scale (mode SCALE:
+ * mag = alpha * min); the loss is the mean over iterations and bits of
+ * softplus(posterior / 8) on all-zero codewords.  Quantised tables drive
+ * LDPC_LOMS (offsets in LSB) and LDPC_LNMS (round(8 * alpha)). */
+enum { LDPC_LEARN_OFFSET = 0, LDPC_LEARN_SCALE = 1 };
+typedef struct ldpc_trainer ldpc_trainer;
+ldpc_trainer *ldpc_trainer_create(const ldpc_code *code, int mode, int iterations, int n_degs,
+                                  size_t batch_frames);
+void ldpc_trainer_destroy(ldpc_trainer *tr);
+/* Loss and gradient [iterations][n_degs] for `params` on device LLRs
+ * (frame-major, batch_frames frames; NULL = the trainer's own channel
+ * buffer).  grad and d_post_out (posterior after the last iteration,
+ * [frames][N] floats on the device) may be NULL. */
+int ldpc_trainer_loss_grad(ldpc_trainer *tr, const float *params, const int8_t *d_llr, float *loss,
+                           float *grad, float *d_post_out);
+typedef struct {
+    int steps;                 /* Adam steps, one fresh batch each              */
+    double lr, beta1, beta2, eps;
+    double ebn0_lo, ebn0_hi;   /* per-frame Eb/N0 range (dB)                    */
+    double rate;               /* <= 0: (N - #checks) / N                       */
+    uint64_t seed, first_frame;
+    int quant;                 /* channel quantizer (0 or 1)                    */
+    double pmin, pmax;         /* parameter box                                 */
+} ldpc_learn_config;
+/* Adam on `params` (in/out); loss_history[steps] may be NULL. */
+int ldpc_trainer_fit(ldpc_trainer *tr, const ldpc_learn_config *cfg, float *params,
+                     float *loss_history);
+
 /* ---- device memory helpers (plumbing for callers without a HIP runtime
  * binding; replaces CUDA_MALLOC_DEVICE/CUDA_MALLOC_HOST,
  * src/custom_api/custom_cuda.h:24-40, and CTrame's pinned buffers,

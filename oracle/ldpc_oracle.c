@@ -395,6 +395,21 @@ void orc_channel_frames(int8_t *out, int N, size_t f0, size_t nf, uint64_t seed,
                 orc_llr(seed, snr_idx, (f0 + f) * (uint64_t)N + (uint64_t)v, sigma, quant);
 }
 
+void orc_channel_frames_range(int8_t *out, int N, size_t f0, size_t nf, uint64_t seed,
+                              uint32_t snr_idx, double lo_db, double hi_db, double rate, int quant) {
+    for (size_t f = 0; f < nf; f++) {
+        const uint64_t fi = f0 + f;
+        uint32_t ctr[4] = {(uint32_t)fi, (uint32_t)(fi >> 32), snr_idx, 0x534E5252u};
+        uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+        uint32_t r[4];
+        orc_philox4x32_10(ctr, key, r);
+        const double ebn0 = lo_db + (hi_db - lo_db) * ((double)r[0] * 0x1.0p-32);
+        const double sigma = sqrt(0.5 / (rate * pow(10.0, ebn0 / 10.0)));
+        for (int v = 0; v < N; v++)
+            out[f * (size_t)N + (size_t)v] = orc_llr(seed, snr_idx, fi * (uint64_t)N + (uint64_t)v, sigma, quant);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* error counting: src/ber_analyzer/CErrorAnalyzer.cpp:141-157 (counts the   */
 /* first _vars = N - K positions of each frame, all-zero codeword)           */
@@ -427,4 +442,158 @@ void orc_syndrome(const uint8_t *hard, int N, int C, const int *deg,
         }
         ok[f] = (uint8_t)good;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* float shadow decoder: forward (stores pre-clamp v2c values r and the      */
+/* posterior after each iteration) and reverse-mode gradient w.r.t. the      */
+/* per-iteration, per-degree table.  Reverse sweep: checks in reverse table   */
+/* order; the gradient reaching a message goes to the edge holding min1       */
+/* (or min2 for the minimum edge itself).                                     */
+/* ------------------------------------------------------------------------ */
+static float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static float softplusf(float x) { return (x > 0 ? x : 0.0f) + log1pf(expf(-fabsf(x))); }
+static float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+/* recompute one check from its pre-clamp values r[0..d) */
+static void shadow_check(int d, const float *r, float par_val, int mode, float *t, float *mnew,
+                         int *i1o, int *i2o, float *min1o, float *min2o) {
+    float min1 = 127.0f, min2 = 127.0f;
+    int i1 = -1, i2 = -1, par = 0;
+    for (int j = 0; j < d; j++) {
+        t[j] = clampf(r[j], -128.0f, 127.0f);
+        float a = fabsf(t[j]);
+        if (a < min1) { min2 = min1; i2 = i1; min1 = a; i1 = j; }
+        else if (a < min2) { min2 = a; i2 = j; }
+        par ^= t[j] > 0.0f;
+    }
+    for (int j = 0; j < d; j++) {
+        float mo = (j == i1) ? min2 : min1;
+        float mag = mode == 0 ? clampf(mo - par_val, 0.0f, 31.0f) : par_val * mo;
+        int pos_others = par ^ (t[j] > 0.0f);
+        mnew[j] = pos_others ? mag : -mag;
+    }
+    *i1o = i1; *i2o = i2; *min1o = min1; *min2o = min2;
+}
+
+static double shadow_run(int N, int C, const int *deg, const uint32_t *pos, int mode, int T,
+                         int dcols, const float *params, const int8_t *llr, size_t F, float *grad,
+                         float *post_out);
+
+double orc_shadow_loss_grad(int N, int C, const int *deg, const uint32_t *pos, int mode, int T,
+                            int dcols, const float *params, const int8_t *llr, size_t F,
+                            float *grad) {
+    return shadow_run(N, C, deg, pos, mode, T, dcols, params, llr, F, grad, NULL);
+}
+
+double orc_shadow_forward(int N, int C, const int *deg, const uint32_t *pos, int mode, int T,
+                          int dcols, const float *params, const int8_t *llr, size_t F,
+                          float *post_out) {
+    return shadow_run(N, C, deg, pos, mode, T, dcols, params, llr, F, NULL, post_out);
+}
+
+static double shadow_run(int N, int C, const int *deg, const uint32_t *pos, int mode, int T,
+                         int dcols, const float *params, const int8_t *llr, size_t F, float *grad,
+                         float *post_out) {
+    size_t *eoff = (size_t *)malloc(sizeof(size_t) * (size_t)(C + 1));
+    eoff[0] = 0;
+    for (int c = 0; c < C; c++) eoff[c + 1] = eoff[c] + (size_t)deg[c];
+    const size_t E = eoff[C];
+    float *p = (float *)malloc(sizeof(float) * (size_t)N);
+    float *m = (float *)malloc(sizeof(float) * E);
+    float *r = (float *)malloc(sizeof(float) * E * (size_t)T);
+    float *pend = (float *)malloc(sizeof(float) * (size_t)N * (size_t)T);
+    float *gp = (float *)malloc(sizeof(float) * (size_t)N);
+    float *gm = (float *)malloc(sizeof(float) * E);
+    if (grad) memset(grad, 0, sizeof(float) * (size_t)T * (size_t)dcols);
+    double loss = 0.0;
+    float t[ORC_MAXD], mn[ORC_MAXD], ga[ORC_MAXD], gt[ORC_MAXD];
+    const float wl = 1.0f / ((float)T * (float)N);
+    for (size_t f = 0; f < F; f++) {
+        for (int v = 0; v < N; v++) p[v] = (float)llr[f * (size_t)N + (size_t)v];
+        memset(m, 0, sizeof(float) * E);
+        double lf = 0.0;
+        for (int it = 0; it < T; it++) {
+            float *ri = r + (size_t)it * E;
+            for (int c = 0; c < C; c++) {
+                const int d = deg[c];
+                const size_t e0 = eoff[c];
+                const float pv = params[(size_t)it * dcols + (d < dcols ? d : dcols - 1)];
+                for (int j = 0; j < d; j++) ri[e0 + j] = p[pos[e0 + j]] - (it == 0 ? 0.0f : m[e0 + j]);
+                int i1, i2;
+                float mi1, mi2;
+                shadow_check(d, ri + e0, pv, mode, t, mn, &i1, &i2, &mi1, &mi2);
+                for (int j = 0; j < d; j++) {
+                    m[e0 + j] = mn[j];
+                    p[pos[e0 + j]] = clampf(t[j] + mn[j], -128.0f, 127.0f);
+                }
+            }
+            for (int v = 0; v < N; v++) {
+                pend[(size_t)it * N + v] = p[v];
+                lf += softplusf(p[v] / 8.0f);
+            }
+        }
+        loss += lf * wl;
+        if (post_out) memcpy(post_out + f * (size_t)N, p, sizeof(float) * (size_t)N);
+        if (!grad) continue;
+        memset(gp, 0, sizeof(float) * (size_t)N);
+        memset(gm, 0, sizeof(float) * E);
+        for (int it = T - 1; it >= 0; it--) {
+            for (int v = 0; v < N; v++) gp[v] += wl * sigmoidf(pend[(size_t)it * N + v] / 8.0f) / 8.0f;
+            const float *ri = r + (size_t)it * E;
+            for (int c = C - 1; c >= 0; c--) {
+                const int d = deg[c];
+                const size_t e0 = eoff[c];
+                const int col = d < dcols ? d : dcols - 1;
+                const float pv = params[(size_t)it * dcols + col];
+                int i1, i2;
+                float mi1, mi2;
+                shadow_check(d, ri + e0, pv, mode, t, mn, &i1, &i2, &mi1, &mi2);
+                int par = 0;
+                for (int k = 0; k < d; k++) par ^= t[k] > 0.0f;
+                for (int j = 0; j < d; j++) ga[j] = 0.0f;
+                for (int j = 0; j < d; j++) {
+                    /* p_j' = clamp(t_j + m_j): gradient passes inside the clamp */
+                    const float u = t[j] + mn[j];
+                    const float gu = (u > -128.0f && u < 127.0f) ? gp[pos[e0 + j]] : 0.0f;
+                    gt[j] = gu;
+                    /* m_j feeds p_j' and, through gm, the next iteration's t_j */
+                    const float gmn = gu + gm[e0 + j];
+                    const float s = (par ^ (t[j] > 0.0f)) ? 1.0f : -1.0f;
+                    const float gmag = s * gmn;
+                    const float mo = (j == i1) ? mi2 : mi1;
+                    float gmo;
+                    if (mode == 0) {
+                        const float x = mo - pv;
+                        const int inside = x > 0.0f && x < 31.0f;
+                        gmo = inside ? gmag : 0.0f;
+                        grad[(size_t)it * dcols + col] -= inside ? gmag : 0.0f;
+                    } else {
+                        gmo = gmag * pv;
+                        grad[(size_t)it * dcols + col] += gmag * mo;
+                    }
+                    /* mo is |t| of the edge holding min1, or min2 for that edge itself */
+                    if (j == i1) {
+                        if (i2 >= 0) ga[i2] += gmo;
+                    } else if (i1 >= 0) {
+                        ga[i1] += gmo;
+                    }
+                }
+                for (int k = 0; k < d; k++) {
+                    const float sg = t[k] > 0.0f ? 1.0f : (t[k] < 0.0f ? -1.0f : 0.0f);
+                    gt[k] += ga[k] * sg;
+                }
+                for (int j = 0; j < d; j++) {
+                    const float rj = ri[e0 + j];
+                    const int inside = rj > -128.0f && rj < 127.0f;
+                    gp[pos[e0 + j]] = inside ? gt[j] : 0.0f;
+                    gm[e0 + j] = (it > 0 && inside) ? -gt[j] : 0.0f;
+                }
+            }
+        }
+    }
+    if (grad)
+        for (int i = 0; i < T * dcols; i++) grad[i] /= (float)F;
+    free(p); free(m); free(r); free(pend); free(gp); free(gm); free(eoff);
+    return F ? loss / (double)F : 0.0;
 }

@@ -94,6 +94,11 @@ int8_t orc_llr(uint64_t seed, uint32_t snr_idx, uint64_t s, double sigma, int qu
 void orc_channel_frames(int8_t *out, int N, size_t f0, size_t nf, uint64_t seed,
                         uint32_t snr_idx, double sigma, int quant);
 
+/* Frames [f0, f0+nf) with per-frame Eb/N0 = lo + (hi-lo)*u(f), u from the
+ * Philox block (f_lo, f_hi, snr_idx, 0x534E5252), sigma = sqrt(0.5/(R 10^(EbN0/10))). */
+void orc_channel_frames_range(int8_t *out, int N, size_t f0, size_t nf, uint64_t seed,
+                              uint32_t snr_idx, double lo_db, double hi_db, double rate, int quant);
+
 /* ---- error counting (src/ber_analyzer/CErrorAnalyzer.cpp:141-157) -------- */
 /* hard[f*N + v]; counts positions v < n_info. Returns frames in error.       */
 long long orc_count_errors(const uint8_t *hard, int N, size_t frames, int n_info,
@@ -101,6 +106,26 @@ long long orc_count_errors(const uint8_t *hard, int N, size_t frames, int n_info
 /* Per frame syndrome check: 1 = all parity checks satisfied.                */
 void orc_syndrome(const uint8_t *hard, int N, int C, const int *deg,
                   const uint32_t *pos, size_t frames, uint8_t *ok);
+
+/* ---- float shadow of the layered decoder (learned tables) --------------
+ * Same serial table order as orc_decode, in float LSB units:
+ *   t = clamp(p - m, -128, 127)   (first iteration: m = 0)
+ *   mo_j = min of the OTHER |t| (127 cap), sign by parity of the others' t > 0
+ *   mode 0 (offset): mag = clamp(mo - beta[it][d], 0, 31)
+ *   mode 1 (scale):  mag = alpha[it][d] * mo
+ *   m_j = +-mag, p = clamp(t + m_j, -128, 127)
+ * loss per frame = mean over iterations and variables of softplus(p / 8)
+ * (all-zero codeword: a positive value votes for bit 1).
+ * params / grad: [T][dcols], column = min(degree, dcols - 1).
+ * Returns the mean loss over F frames; grad (may be NULL) is its gradient. */
+double orc_shadow_loss_grad(int N, int C, const int *deg, const uint32_t *pos, int mode, int T,
+                            int dcols, const float *params, const int8_t *llr, size_t F,
+                            float *grad);
+/* Same forward pass; writes each frame's posterior after T iterations
+ * (post_out[f*N + v]) and returns the mean loss. */
+double orc_shadow_forward(int N, int C, const int *deg, const uint32_t *pos, int mode, int T,
+                          int dcols, const float *params, const int8_t *llr, size_t F,
+                          float *post_out);
 
 #ifdef __cplusplus
 }

@@ -49,8 +49,16 @@ def lib() -> C.CDLL:
         L.orc_channel_frames.restype = None
         L.orc_channel_frames.argtypes = [_i8_p, C.c_int, C.c_size_t, C.c_size_t, C.c_uint64, C.c_uint32,
                                          C.c_double, C.c_int]
+        L.orc_channel_frames_range.restype = None
+        L.orc_channel_frames_range.argtypes = [_i8_p, C.c_int, C.c_size_t, C.c_size_t, C.c_uint64, C.c_uint32,
+                                               C.c_double, C.c_double, C.c_double, C.c_int]
         L.orc_count_errors.restype = C.c_longlong
         L.orc_count_errors.argtypes = [_u8_p, C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_longlong), _u32_p]
+        L.orc_shadow_loss_grad.restype = C.c_double
+        L.orc_shadow_loss_grad.argtypes = [C.c_int, C.c_int, _i_p, _u32_p, C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(C.c_float), _i8_p, C.c_size_t, C.POINTER(C.c_float)]
+        L.orc_shadow_forward.restype = C.c_double
+        L.orc_shadow_forward.argtypes = L.orc_shadow_loss_grad.argtypes
         L.orc_syndrome.restype = None
         L.orc_syndrome.argtypes = [_u8_p, C.c_int, C.c_int, _i_p, _u32_p, C.c_size_t, _u8_p]
         _lib = L
@@ -154,6 +162,12 @@ def channel_frames(n: int, f0: int, nf: int, seed: int, snr_idx: int, sig: float
     return out
 
 
+def channel_frames_range(n, f0, nf, seed, snr_idx, lo, hi, rate=0.5, quant=0):
+    out = np.empty(nf * n, np.int8)
+    lib().orc_channel_frames_range(_p(out, C.c_int8), n, f0, nf, seed, snr_idx, lo, hi, rate, quant)
+    return out
+
+
 def count_errors(hard: np.ndarray, n: int, n_info: int):
     hard = np.ascontiguousarray(hard, dtype=np.uint8)
     frames = hard.size // n
@@ -172,3 +186,30 @@ def syndrome_ok(hard: np.ndarray, n: int, deg, pos) -> np.ndarray:
     lib().orc_syndrome(_p(hard, C.c_uint8), n, deg.size, _p(deg, C.c_int), _p(pos, C.c_uint32), frames,
                        _p(ok, C.c_uint8))
     return ok
+
+
+def shadow_loss_grad(deg, pos, n, mode, params, llr, want_grad=True):
+    """Float shadow decoder loss (and gradient) over frame-major int8 LLRs."""
+    deg = np.ascontiguousarray(deg, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.uint32)
+    prm = np.ascontiguousarray(params, dtype=np.float32)
+    llr = np.ascontiguousarray(llr, dtype=np.int8).reshape(-1)
+    T, dcols = prm.shape
+    grad = np.zeros_like(prm) if want_grad else None
+    loss = lib().orc_shadow_loss_grad(n, deg.size, _p(deg, C.c_int), _p(pos, C.c_uint32), mode, T, dcols,
+                                      _p(prm, C.c_float), _p(llr, C.c_int8), llr.size // n,
+                                      _p(grad, C.c_float) if want_grad else None)
+    return float(loss), grad
+
+
+def shadow_forward(deg, pos, n, mode, params, llr):
+    """Float shadow decoder: (mean loss, final posterior [frames][n])."""
+    deg = np.ascontiguousarray(deg, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.uint32)
+    prm = np.ascontiguousarray(params, dtype=np.float32)
+    llr = np.ascontiguousarray(llr, dtype=np.int8).reshape(-1)
+    F = llr.size // n
+    out = np.empty(F * n, np.float32)
+    loss = lib().orc_shadow_forward(n, deg.size, _p(deg, C.c_int), _p(pos, C.c_uint32), mode, prm.shape[0],
+                                    prm.shape[1], _p(prm, C.c_float), _p(llr, C.c_int8), F, _p(out, C.c_float))
+    return float(loss), out.reshape(F, n)
