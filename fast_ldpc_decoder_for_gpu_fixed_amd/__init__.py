@@ -338,6 +338,63 @@ class DeviceBuffer:
             self.ptr = None
 
 
+class PoolDecoder:
+    """Persistent frame-queue decoder: each workgroup keeps 4P frames
+    resident, stops every frame on its own (every check with an even parity
+    of positive v2c values, or max_iterations) and refills its slot from a
+    device queue.  Inputs and outputs stay on the device (frame-major)."""
+
+    def __init__(self, code: Code, variant: int, max_iterations: int, words_per_block: int = 0,
+                 threads: int = 0, table=None):
+        tu = Tuning(words_per_block, threads, -1, -1, -1, 0)
+        h = load().ldpc_pool_create(code.handle, variant, max_iterations, C.byref(tu))
+        if not h:
+            raise LdpcError(f"ldpc_pool_create failed: {_lib.last_error()}")
+        self._h = C.c_void_p(h)
+        self.code, self.variant, self.max_iterations = code, variant, max_iterations
+        if table is not None:
+            self.set_table(table)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.ldpc_pool_destroy(h)
+            self._h = None
+
+    def set_table(self, table) -> None:
+        t = np.ascontiguousarray(table, dtype=np.int8)
+        assert t.ndim == 2
+        check(load().ldpc_pool_set_table(self._h, ptr(t, C.c_int8), t.shape[0], t.shape[1]), "pool_set_table")
+
+    def decode_device(self, d_in: int, frames: int, d_out: int, d_iters: int = 0, workgroups: int = 0,
+                      stream: int = 0) -> None:
+        check(load().ldpc_pool_decode(self._h, C.c_void_p(d_in), frames, C.c_void_p(d_out),
+                                      C.c_void_p(d_iters) if d_iters else None, workgroups,
+                                      C.c_void_p(stream) if stream else None), "pool_decode")
+
+    def decode(self, llr: np.ndarray, workgroups: int = 0):
+        """Host convenience: frame-major int8 LLRs -> (decisions [F][N], iterations [F])."""
+        n = self.code.n_vars
+        llr = np.ascontiguousarray(llr, dtype=np.int8).reshape(-1, n)
+        F = llr.shape[0]
+        d_in, d_out, d_it = DeviceBuffer(llr.nbytes), DeviceBuffer(F * n), DeviceBuffer(4 * F)
+        d_in.upload(llr)
+        self.decode_device(d_in.ptr, F, d_out.ptr, d_it.ptr, workgroups)
+        synchronize()
+        hard = d_out.download(np.empty((F, n), np.uint8))
+        its = d_it.download(np.empty(F, np.int32))
+        return hard, its
+
+    @property
+    def stream(self) -> int:
+        return int(load().ldpc_pool_stream(self._h) or 0)
+
+    def config(self) -> dict:
+        P, T, G = C.c_int(), C.c_int(), C.c_int()
+        check(load().ldpc_pool_config(self._h, C.byref(P), C.byref(T), C.byref(G)), "pool_config")
+        return {"words_per_block": P.value, "threads": T.value, "max_workgroups": G.value}
+
+
 def channel_sigma(snr_db: float, rate: float, es_n0: bool = False) -> float:
     return float(load().ldpc_channel_sigma(snr_db, rate, int(es_n0)))
 

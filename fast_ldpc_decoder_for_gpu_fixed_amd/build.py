@@ -30,7 +30,8 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["decode_kernels.hip", "sim_kernels.hip", "learn_kernels.hip", "api.hip", "learn_api.hip", "code.cpp"]
+SOURCES = ["decode_kernels.hip", "sim_kernels.hip", "learn_kernels.hip", "pool_kernels.hip", "api.hip",
+           "learn_api.hip", "pool_api.hip", "code.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -1,0 +1,151 @@
+// check_core.h -- the check-node update on one word (4 frames) in packed-16
+// byte lanes, shared by the per-batch and the persistent decode kernels.
+//
+// What is computed (per frame, the reference's serial table order;
+// src/decoder_oms/cuda/CUDA_OMS_SIMD.cu:148-191):
+//     t_j  = sat8(post[v_j] - msg_j)            (first iteration: post[v_j])
+//     min1, min2 = two smallest |t_j| (unsigned, |-128| = 128, both start at 127)
+//     msg_j = +-(|t_j| == min1 ? f(min2) : g(min1)),  sign = parity of the others' t > 0
+//     post[v_j] = sat8(t_j + msg_j)
+#pragma once
+#include "byte_lanes.h"
+#include "kernels.h"
+
+namespace ldpc {
+
+// learned parameter of one frame in the form magnitudes() takes
+__device__ __forceinline__ uint32_t learned_half(int variant, int v) {
+    return variant == VAR_LNMS ? ((uint32_t)v << 5) : ((uint32_t)v << 8);
+}
+
+// ---------------------------------------------------------------------------
+// per-check magnitudes  (cste_1 from min2, cste_2 from min1)
+// ---------------------------------------------------------------------------
+// lp: learned parameter of each frame of the half (LOMS: beta << 8, LNMS:
+// alpha << 5), so frames of one word may sit at different iterations.
+__device__ __forceinline__ void magnitudes(int variant, u16x2 min1, u16x2 min2, bool nocap, u16x2 lp,
+                                           u16x2 &k1, u16x2 &k2) {
+    const u16x2 cap31 = splat(0x1F00u);
+    switch (variant) {
+    case VAR_MS:  // CUDA_MS_SIMD.cu:173-174
+        k1 = umin(min2, cap31);
+        k2 = umin(min1, cap31);
+        break;
+    case VAR_OMS:      // CUDA_OMS_SIMD.cu:173-174; no cap: :115-116
+    case VAR_OMS_V2: {  // CUDA_OMS_SIMD_v2.cu:145-146
+        const u16x2 cap = nocap ? splat(0x7F00u) : cap31;
+        const u16x2 one = splat(0x0100u);
+        k1 = umin(sub_sat_u(min2, one), cap);
+        k2 = umin(sub_sat_u(min1, one), cap);
+        break;
+    }
+    case VAR_LOMS:  // learned offset beta (LSB), cap 31
+        k1 = umin(sub_sat_u(min2, lp), cap31);
+        k2 = umin(sub_sat_u(min1, lp), cap31);
+        break;
+    case VAR_NMS: {  // (char)(0.75f*m) = floor(3m/4): m<<8 - m<<6 keeps 3m/4 exactly
+        const u16x2 hi = splat(0xFF00u);
+        k1 = (min2 - (min2 >> (u16x2)(2))) & hi;
+        k2 = (min1 - (min1 >> (u16x2)(2))) & hi;
+        break;
+    }
+    case VAR_2NMS: {  // 0.875 on min2, 0.75 on min1 (CUDA_2NMS_SIMD.cu:73-85)
+        const u16x2 hi = splat(0xFF00u);
+        k1 = (min2 - (min2 >> (u16x2)(3))) & hi;
+        k2 = (min1 - (min1 >> (u16x2)(2))) & hi;
+        break;
+    }
+    default: {  // VAR_LNMS: min(floor(m * alpha / 8), 31), alpha in [0, 8]
+        const u16x2 hi = splat(0xFF00u);
+        k1 = umin(((min2 >> (u16x2)(8)) * lp) & hi, cap31);
+        k2 = umin(((min1 >> (u16x2)(8)) * lp) & hi, cap31);
+        break;
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one check of degree D on one word.
+// EQ = false: r = k1 - min(a - min1, k1 - k2).  Valid when k1 - k2 <= min2 - min1
+//   (every 1-Lipschitz magnitude map: MS, OMS, NMS, learned tables): a lane
+//   that is not the minimum has a >= min2.  The residue 0x7FFF (a saturated
+//   +127, or |-128|) only occurs with a >= 127 >= min2, so it lands on k2 as
+//   the reference's lane does, or on k1 == k2 when min1 == min2 == 127.
+// EQ = true (2NMS, k1 - k2 can exceed min2 - min1): t is cleaned to its high
+//   byte and the test a == min1 is made explicit.
+// Returns the parity-of-positives bits of the even (x) and odd (y) frames at
+// 0x80008000: frame 0 = x bit 15, 1 = y bit 15, 2 = x bit 31, 3 = y bit 31.
+// ---------------------------------------------------------------------------
+template <int D, bool EQ>
+__device__ __forceinline__ uint2 check_core(const uint32_t (&pw)[D], const uint32_t (&mw)[D], bool first,
+                                            int variant, bool nocap, u16x2 lpe, u16x2 lpo, uint32_t (&pn)[D],
+                                            uint32_t (&mn)[D]) {
+    constexpr bool KEEP = D <= 12;  // keep |t| and -t live across the two passes
+    u16x2 te[D], to[D], ne[KEEP ? D : 1], no[KEEP ? D : 1], ae[KEEP ? D : 1], ao[KEEP ? D : 1];
+    u16x2 m1e = splat(0x7F00u), m2e = m1e, m1o = m1e, m2o = m1e;
+    u16x2 se = splat(0u), so = se;
+    const u16x2 zero = splat(0u);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        u16x2 xe = unpack_even(pw[j]), xo = unpack_odd(pw[j]);
+        if (!first) {
+            xe = sub_sat_s(xe, unpack_even(mw[j]));
+            xo = sub_sat_s(xo, unpack_odd(mw[j]));
+        }
+        if (EQ) {
+            xe &= splat(0xFF00u);
+            xo &= splat(0xFF00u);
+        }
+        te[j] = xe;
+        to[j] = xo;
+        const u16x2 ye = sub_sat_s(zero, xe), yo = sub_sat_s(zero, xo);  // -t, top bit = (t > 0)
+        const u16x2 be = umin(xe, ye), bo = umin(xo, yo);              // |t| as uint
+        se ^= ye;
+        so ^= yo;
+        m2e = umin(m2e, umax(be, m1e));
+        m1e = umin(m1e, be);
+        m2o = umin(m2o, umax(bo, m1o));
+        m1o = umin(m1o, bo);
+        if constexpr (KEEP) {
+            ne[j] = ye; no[j] = yo; ae[j] = be; ao[j] = bo;
+        }
+    }
+    const uint2 parity = make_uint2(w32(se) & 0x80008000u, w32(so) & 0x80008000u);
+
+    u16x2 k1e, k2e, k1o, k2o;
+    magnitudes(variant, m1e, m2e, nocap, lpe, k1e, k2e);
+    magnitudes(variant, m1o, m2o, nocap, lpo, k1o, k2o);
+    const u16x2 Ke = k1e - k2e, Ko = k1o - k2o;
+    // k1 - min(a - min1, k1 - k2) == max(sat_u(k1 + min1 - a), k2): one op
+    // less per half and edge (k1 + min1 <= 0xFE00 cannot wrap)
+    const u16x2 C1e = k1e + m1e, C1o = k1o + m1o;
+    se ^= splat(0x8000u);  // top bit now: parity of the others is even -> negate
+    so ^= splat(0x8000u);
+
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        u16x2 ye, yo, be, bo;
+        if constexpr (KEEP) {
+            ye = ne[j]; yo = no[j]; be = ae[j]; bo = ao[j];
+        } else {
+            ye = sub_sat_s(zero, te[j]); yo = sub_sat_s(zero, to[j]);
+            be = umin(te[j], ye); bo = umin(to[j], yo);
+        }
+        u16x2 re, ro;
+        if (EQ) {
+            const u16x2 de = be - m1e, dd = bo - m1o;  // >= 0, zero exactly on the minimum
+            re = k2e + sub_sat_u(splat(0x0001u), de) * Ke;
+            ro = k2o + sub_sat_u(splat(0x0001u), dd) * Ko;
+        } else {
+            re = umax(sub_sat_u(C1e, be), k2e);
+            ro = umax(sub_sat_u(C1o, bo), k2o);
+        }
+        const u16x2 se_j = sign_mask(se ^ ye), so_j = sign_mask(so ^ yo);
+        const u16x2 me = (re ^ se_j) - se_j, mo = (ro ^ so_j) - so_j;
+        mn[j] = repack(me, mo);
+        pn[j] = repack(add_sat_s(te[j], me), add_sat_s(to[j], mo));
+    }
+    return parity;
+}
+
+}  // namespace ldpc

@@ -52,6 +52,26 @@ int max_threads_for_degree(int dmax, bool uniform);
 hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipStream_t s);
 hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s);
 
+// persistent frame-queue decoder (pool_kernels.hip)
+struct PoolArgs {
+    const uint2 *checks;
+    const uint32_t *edge_var;
+    const int *level_beg;
+    int n_levels, N, C, E;
+    int P, log2P;
+    const int8_t *in_frames;      // [total][N] frame-major LLRs
+    uint8_t *out_hard;            // [total][N] decisions
+    int *out_iters;               // [total] iterations per frame, or null
+    unsigned long long *next;     // queue head (device counter, starts at 0)
+    unsigned long long total;     // frames in the queue
+    uint32_t *msgs;               // [grid][E][P] per-workgroup message words
+    int max_iters, variant;
+    const int8_t *ltable;         // learned table or null
+    int tits, tdeg;
+};
+hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s);
+int pool_blocks_per_cu(const PoolArgs &a, int dmax, int threads);
+
 // float shadow decoder (learned tables)
 struct ShadowArgs {
     const uint2 *checks;

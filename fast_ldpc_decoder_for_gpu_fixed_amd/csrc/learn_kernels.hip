@@ -5,7 +5,7 @@
 // Forward (per frame, table order = dependency levels, in LSB units):
 //     r = p - m (first iteration: p);  t = clamp(r, -128, 127)
 //     mo_j = min of the other |t| (127 cap);  sign by the others' parity
-//     mag = clamp(mo - beta[it][d], 0, 31)   or   alpha[it][d] * mo
+//     mag = clamp(mo - beta[it][d], 0, 31)   or   clamp(alpha[it][d] * mo, 0, 31)
 //     m_j = +-mag;  p = clamp(t + m_j, -128, 127)
 //     loss += softplus(p / 8) / (T * N)  after every iteration
 // Reverse: the gradient of a message reaches |t| of the edge holding min1
@@ -60,7 +60,7 @@ __device__ __forceinline__ void shadow_eval(const float (&r)[D], float pv, int m
 #pragma unroll
     for (int j = 0; j < D; j++) {
         const float mo = (j == s.i1) ? s.min2 : s.min1;
-        const float mag = mode == 0 ? clampf_(mo - pv, 0.0f, 31.0f) : pv * mo;
+        const float mag = mode == 0 ? clampf_(mo - pv, 0.0f, 31.0f) : clampf_(pv * mo, 0.0f, 31.0f);
         s.mn[j] = (s.par ^ (s.t[j] > 0.0f)) ? mag : -mag;
     }
 }
@@ -123,8 +123,10 @@ __device__ __forceinline__ float bwd_check(const ShadowArgs &a, float *gpost, fl
             gmo = inside ? gmag : 0.0f;
             gparam -= gmo;
         } else {
-            gmo = gmag * pv;
-            gparam += gmag * mo;
+            const float x = pv * mo;
+            const bool inside = x > 0.0f && x < 31.0f;
+            gmo = inside ? gmag * pv : 0.0f;
+            gparam += inside ? gmag * mo : 0.0f;
         }
         const int dst = (j == s.i1) ? s.i2 : s.i1;
 #pragma unroll

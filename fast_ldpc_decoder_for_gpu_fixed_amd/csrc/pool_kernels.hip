@@ -1,0 +1,235 @@
+// pool_kernels.hip -- persistent decoding of a frame queue with per-frame
+// early stop and slot refill.
+//
+// A workgroup keeps 4P frame slots resident: P words of 4 byte lanes, the
+// posterior of all of them in LDS ([var][P] words, as in k_decode).  It runs
+// the layered iteration on all slots at once; after each iteration a frame
+// whose checks all had an even parity of positive v2c values (the
+// reference's stop rule, src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:237-244,
+// applied to each frame on its own) or that reached the iteration limit
+// writes its decisions and iteration count, and its lane is refilled with
+// the next frame of the queue (one atomic per workgroup and iteration).  A
+// fresh lane reads zero messages in its first iteration: the per-word byte
+// mask `keep` clears that lane of every message word it loads.  Learned
+// tables are looked up per lane with the lane's own iteration index.
+// The workgroup exits when the queue is empty and every slot is drained.
+#include "check_core.h"
+
+namespace ldpc {
+
+constexpr int POOL_SLOTS = 4 * MAX_P;
+constexpr int POOL_TABLE = 64 * (MAX_DEG + 1);  // learned table bytes kept in LDS
+
+template <int D, bool EQ>
+__device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__restrict__ ev, uint32_t *__restrict__ mp,
+                                            int lp, int w, uint32_t keep, int variant, u16x2 lpe, u16x2 lpo) {
+    uint32_t vi[D], pw[D], mw[D], pn[D], mn[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) vi[j] = (ev[j] << lp) + (uint32_t)w;
+#pragma unroll
+    for (int j = 0; j < D; j++) pw[j] = post[vi[j]];
+#pragma unroll
+    for (int j = 0; j < D; j++) mw[j] = mp[j] & keep;
+    const uint2 pr = check_core<D, EQ>(pw, mw, false, variant, false, lpe, lpo, pn, mn);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        mp[j] = mn[j];
+        post[vi[j]] = pn[j];
+    }
+    return pr;
+}
+
+template <bool EQ, int DMAXC>
+__device__ __forceinline__ uint2 pool_check_any(int D, uint32_t *post, const uint32_t *__restrict__ ev,
+                                                uint32_t *__restrict__ mp, int lp, int w, uint32_t keep, int variant,
+                                                u16x2 lpe, u16x2 lpo) {
+    switch (D) {
+#define LDPC_PCASE(d)                                                                        \
+    case d:                                                                                  \
+        if constexpr (d <= DMAXC)                                                            \
+            return pool_check<d, EQ>(post, ev, mp, lp, w, keep, variant, lpe, lpo);          \
+        else                                                                                 \
+            return make_uint2(0u, 0u);
+        LDPC_PCASE(1) LDPC_PCASE(2) LDPC_PCASE(3) LDPC_PCASE(4) LDPC_PCASE(5) LDPC_PCASE(6)
+        LDPC_PCASE(7) LDPC_PCASE(8) LDPC_PCASE(9) LDPC_PCASE(10) LDPC_PCASE(11) LDPC_PCASE(12)
+        LDPC_PCASE(13) LDPC_PCASE(14) LDPC_PCASE(15) LDPC_PCASE(16) LDPC_PCASE(17) LDPC_PCASE(18)
+        LDPC_PCASE(19) LDPC_PCASE(20) LDPC_PCASE(21) LDPC_PCASE(22) LDPC_PCASE(23) LDPC_PCASE(24)
+        LDPC_PCASE(25) LDPC_PCASE(26) LDPC_PCASE(27) LDPC_PCASE(28) LDPC_PCASE(29) LDPC_PCASE(30)
+        LDPC_PCASE(31) LDPC_PCASE(32)
+#undef LDPC_PCASE
+    default: return make_uint2(0u, 0u);
+    }
+}
+
+template <bool EQ, int DMAXC>
+__global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool(const PoolArgs a) {
+    extern __shared__ uint32_t lds_post[];
+    __shared__ long long s_fid[POOL_SLOTS];
+    __shared__ int s_it[POOL_SLOTS];
+    __shared__ int s_flag[POOL_SLOTS];  // 1: stop this iteration, 2: refilled
+    __shared__ uint32_t s_keep[MAX_P];
+    __shared__ uint32_t s_ove[MAX_P], s_ovo[MAX_P];
+    __shared__ int8_t s_tab[POOL_TABLE];
+    __shared__ int s_active;
+
+    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P, N = a.N;
+    const int S = 4 * P;
+    const int w = tid & (P - 1);
+    const int cstride = T >> lp;
+    uint32_t *post = lds_post;
+    uint32_t *mbase = a.msgs + (size_t)blockIdx.x * a.E * P;
+    uint8_t *pb = reinterpret_cast<uint8_t *>(post);
+    const bool learned = a.ltable != nullptr;
+    if (learned)
+        for (int i = tid; i < a.tits * a.tdeg; i += T) s_tab[i] = a.ltable[i];
+    if (tid < S) {
+        s_fid[tid] = -1;
+        s_it[tid] = 0;
+        s_flag[tid] = 1;  // every slot starts empty and asks for a frame
+    }
+    if (tid < P) s_keep[tid] = 0xFFFFFFFFu;
+    __syncthreads();
+
+    for (;;) {
+        // ---- refill: every slot flagged 1 takes the next frame of the queue
+        if (tid == 0) {
+            int n = 0;
+            for (int s = 0; s < S; s++) n += s_flag[s] == 1;
+            unsigned long long base = n ? atomicAdd(a.next, (unsigned long long)n) : 0ull;
+            int active = 0;
+            for (int s = 0; s < S; s++) {
+                if (s_flag[s] == 1) {
+                    if (base < a.total) {
+                        s_fid[s] = (long long)base;
+                        s_it[s] = 0;
+                        s_flag[s] = 2;
+                        s_keep[s >> 2] &= ~(0xFFu << (8 * (s & 3)));
+                    } else {
+                        s_fid[s] = -1;
+                        s_flag[s] = 0;
+                    }
+                    base++;
+                } else {
+                    s_flag[s] = 0;
+                }
+                active += s_fid[s] >= 0;
+            }
+            s_active = active;
+        }
+        __syncthreads();
+        if (s_active == 0) break;
+        // ---- load the LLRs of the refilled frames into their lanes
+        for (int s = 0; s < S; s++) {
+            if (s_flag[s] != 2) continue;  // uniform: flags are read after the barrier
+            const int8_t *src = a.in_frames + (size_t)s_fid[s] * N;
+            const int wl = s >> 2, k = s & 3;
+            for (int v = tid; v < N; v += T) pb[((size_t)v * P + wl) * 4 + k] = (uint8_t)src[v];
+        }
+        if (tid < P) {
+            s_ove[tid] = 0;
+            s_ovo[tid] = 0;
+        }
+        __syncthreads();
+
+        // ---- one layered iteration over all slots
+        const uint32_t keep = s_keep[w];
+        int rowoff[4] = {0, 0, 0, 0};
+        if (learned)
+#pragma unroll
+            for (int k = 0; k < 4; k++) rowoff[k] = min(s_it[4 * w + k], a.tits - 1) * a.tdeg;
+        uint32_t ove = 0, ovo = 0;
+        for (int L = 0; L < a.n_levels; L++) {
+            const int lb = a.level_beg[L], le = a.level_beg[L + 1];
+            for (int c = lb + (tid >> lp); c < le; c += cstride) {
+                const uint2 ci = a.checks[c];
+                const int D = (int)(ci.x & 0xFFu);
+                u16x2 lpe = splat(0u), lpo = splat(0u);
+                if (learned) {
+                    const int col = min(D, a.tdeg - 1);
+                    const uint32_t l0 = learned_half(a.variant, s_tab[rowoff[0] + col]);
+                    const uint32_t l1 = learned_half(a.variant, s_tab[rowoff[1] + col]);
+                    const uint32_t l2 = learned_half(a.variant, s_tab[rowoff[2] + col]);
+                    const uint32_t l3 = learned_half(a.variant, s_tab[rowoff[3] + col]);
+                    lpe = u2(l0 | (l2 << 16));
+                    lpo = u2(l1 | (l3 << 16));
+                }
+                const uint2 pr = pool_check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y,
+                                                           mbase + (size_t)ci.y * P + (size_t)w * D, lp, w, keep,
+                                                           a.variant, lpe, lpo);
+                ove |= pr.x;
+                ovo |= pr.y;
+            }
+            __syncthreads();
+        }
+        if (ove) atomicOr(&s_ove[w], ove);
+        if (ovo) atomicOr(&s_ovo[w], ovo);
+        __syncthreads();
+
+        // ---- per-frame stop decision
+        if (tid < S && s_fid[tid] >= 0) {
+            const int wl = tid >> 2, k = tid & 3;
+            const uint32_t src = (k & 1) ? s_ovo[wl] : s_ove[wl];
+            const bool odd = (src >> ((k & 2) ? 31 : 15)) & 1u;
+            const int it = ++s_it[tid];
+            s_flag[tid] = (!odd || it >= a.max_iters) ? 1 : 0;
+        }
+        if (tid < P) s_keep[tid] = 0xFFFFFFFFu;
+        __syncthreads();
+
+        // ---- write the stopped frames: decisions (post > 0) and iteration count
+        for (int s = 0; s < S; s++) {
+            if (s_flag[s] != 1 || s_fid[s] < 0) continue;
+            const size_t f = (size_t)s_fid[s];
+            const int wl = s >> 2, k = s & 3;
+            uint8_t *dst = a.out_hard + f * N;
+            if ((N & 3) == 0) {
+                uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
+                for (int q = tid; q < (N >> 2); q += T) {
+                    uint32_t x = 0;
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        x |= (uint32_t)((int8_t)pb[((size_t)(4 * q + r) * P + wl) * 4 + k] > 0) << (8 * r);
+                    d32[q] = x;
+                }
+            } else {
+                for (int v = tid; v < N; v += T) dst[v] = (uint8_t)((int8_t)pb[((size_t)v * P + wl) * 4 + k] > 0);
+            }
+            if (tid == 0 && a.out_iters) a.out_iters[f] = s_it[s];
+        }
+        __syncthreads();
+    }
+}
+
+template <bool EQ, int DMAXC>
+static hipError_t launch_pool_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pool<EQ, DMAXC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(MAX_LDS_BYTES - 4096));
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_pool<EQ, DMAXC>), dim3(grid), dim3(threads), (size_t)a.N * a.P * 4, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s) {
+    if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax, false)) return hipErrorInvalidValue;
+    const bool eq = a.variant == VAR_2NMS;
+    if (dmax <= 8) return eq ? launch_pool_t<true, 8>(a, grid, threads, s) : launch_pool_t<false, 8>(a, grid, threads, s);
+    if (dmax <= 16) return eq ? launch_pool_t<true, 16>(a, grid, threads, s) : launch_pool_t<false, 16>(a, grid, threads, s);
+    return eq ? launch_pool_t<true, 32>(a, grid, threads, s) : launch_pool_t<false, 32>(a, grid, threads, s);
+}
+
+int pool_blocks_per_cu(const PoolArgs &a, int dmax, int threads) {
+    int n = 0;
+    const void *f = nullptr;
+    const bool eq = a.variant == VAR_2NMS;
+    if (dmax <= 8) f = eq ? (const void *)k_pool<true, 8> : (const void *)k_pool<false, 8>;
+    else if (dmax <= 16) f = eq ? (const void *)k_pool<true, 16> : (const void *)k_pool<false, 16>;
+    else f = eq ? (const void *)k_pool<true, 32> : (const void *)k_pool<false, 32>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, (size_t)a.N * a.P * 4) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
+}
+
+}  // namespace ldpc

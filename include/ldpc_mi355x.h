@@ -56,7 +56,7 @@ enum {
     LDPC_NMS = 3,    /* CGPU_Decoder_NMS_SIMD   src/decoder_nms/CGPU_Decoder_NMS_SIMD.cu    */
     LDPC_2NMS = 4,   /* CGPU_Decoder_2NMS_SIMD  src/decoder_2nms/CGPU_Decoder_2NMS_SIMD.cu  */
     LDPC_LOMS = 5,   /* learned-offset OMS (int8 table [iteration][check degree])   */
-    LDPC_LNMS = 6    /* learned-normalised MS (table value a: floor(m*a/8))          */
+    LDPC_LNMS = 6    /* learned-normalised MS (table value a: min(floor(m*a/8), 31)) */
 };
 
 /* ---- parity-check codes (replaces the compile-time tables of src/matrix:
@@ -148,8 +148,35 @@ int ldpc_decoder_config(const ldpc_decoder *dec, int *words_per_block, int *thre
 
 /* Learned table for LDPC_LOMS / LDPC_LNMS: int8 [n_iters][n_degs], row =
  * absolute iteration (last row repeats), column = check degree (last column
- * repeats).  LOMS: offset in LSB, 0..127. LNMS: numerator over 8, 0..8. */
+ * repeats).  LOMS: offset in LSB, 0..127. LNMS: numerator over 8, 0..8,
+ * magnitude capped at 31. */
 int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, int n_degs);
+
+/* ---- persistent frame-queue decoder ("pool"; no single counterpart in the
+ * reference: it generalises the per-word early stop of
+ * src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:237-244 to each frame and keeps the
+ * GPU full by refilling a stopped frame's lane from a device queue) --------
+ * A frame stops after the first iteration in which every check has an even
+ * parity of positive v2c values, or at max_iterations; its decisions and
+ * iteration count are written and its slot takes the next queued frame.
+ * Variants: MS, OMS_V2, NMS, 2NMS, LOMS, LNMS, and OMS on codes with a single
+ * degree group (the OMS first-iteration quirk is not per-frame).  Results
+ * per frame equal a one-frame layered decode with that stop rule. */
+typedef struct ldpc_pool ldpc_pool;
+ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations,
+                            const ldpc_tuning *tuning);
+int ldpc_pool_set_table(ldpc_pool *pool, const int8_t *table, int n_iters, int n_degs);
+/* Decode `frames` frame-major LLRs at d_in into d_out (decisions) and
+ * d_iters (iterations per frame, may be NULL).  workgroups = 0: as many as
+ * stay resident on the GPU (capped by the work).  Asynchronous on `stream`
+ * (NULL = the pool's own); the queue counter is reset with a kernel, so the
+ * call may be captured in a HIP graph. */
+int ldpc_pool_decode(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters,
+                     int workgroups, void *stream);
+/* Words per workgroup, workgroup size and resident workgroups chosen. */
+int ldpc_pool_config(const ldpc_pool *pool, int *words_per_block, int *threads, int *max_workgroups);
+void *ldpc_pool_stream(ldpc_pool *pool);
+void ldpc_pool_destroy(ldpc_pool *pool);
 
 /* ---- state API (per-iteration parity; the reference keeps this state in
  * device_V / d_MSG_C_2_V, src/decoder_template/CGPUDecoder.h:21-23) ------- */
@@ -226,7 +253,7 @@ int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, u
  * (no counterpart in the reference).  The shadow runs the integer decoder's
  * layered schedule in float LSB units with a per-iteration, per-check-degree
  * offset (mode OFFSET: mag = clamp(min - beta, 0, 31)) or scale (mode SCALE:
- * mag = alpha * min); the loss is the mean over iterations and bits of
+ * mag = clamp(alpha * min, 0, 31)); the loss is the mean over iterations and bits of
  * softplus(posterior / 8) on all-zero codewords.  Quantised tables drive
  * LDPC_LOMS (offsets in LSB) and LDPC_LNMS (round(8 * alpha)). */
 enum { LDPC_LEARN_OFFSET = 0, LDPC_LEARN_SCALE = 1 };

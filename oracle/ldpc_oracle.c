@@ -145,11 +145,12 @@ static void constants(int variant, uint32_t min1, uint32_t min2, int nocap,
         *c2 = orc_vminu4(orc_vsubus4(min1, b4), 0x1F1F1F1Fu);
         break;
     }
-    case ORC_LNMS: { /* learned normalisation: floor(m * a / 8), a in [0,8] */
+    case ORC_LNMS: { /* learned normalisation: min(floor(m * a / 8), 31), a in [0,8] */
         uint32_t r1 = 0, r2 = 0;
         for (int k = 0; k < 4; k++) {
-            r1 = put(r1, k, (ub(min2, k) * lparam) >> 3);
-            r2 = put(r2, k, (ub(min1, k) * lparam) >> 3);
+            int x1 = (ub(min2, k) * lparam) >> 3, x2 = (ub(min1, k) * lparam) >> 3;
+            r1 = put(r1, k, x1 < 31 ? x1 : 31);
+            r2 = put(r2, k, x2 < 31 ? x2 : 31);
         }
         *c1 = r1;
         *c2 = r2;
@@ -297,6 +298,81 @@ int orc_decode_mt(int N, int C, const int *deg, const uint32_t *pos, int variant
                    W * (size_t)t / (size_t)threads, W * (size_t)(t + 1) / (size_t)threads};
         jobs[t] = j;
         pthread_create(&th[t], NULL, job_run, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+    free(eoff);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* per-frame stop: the stop test of CUDA_2NMS_SIMD.cu:237-244 (every check   */
+/* has an even parity of positive v2c values) applied to each byte lane on   */
+/* its own and after every iteration, the first included.  Lanes never mix  */
+/* (every operation above is per byte), so four frames share a word and a    */
+/* frame's result is its decisions and iteration count when its lane stops.  */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int N, C; const int *deg; const uint32_t *pos; const size_t *eoff; int variant;
+    const int8_t *llr; size_t F; int max_iters; const int8_t *table; int tits, tdeg;
+    uint8_t *hard; int *iters; size_t g0, g1;
+} fjob_t;
+
+static void frames_range(const fjob_t *j) {
+    const int N = j->N;
+    uint32_t *post = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)N);
+    uint32_t *msg = (uint32_t *)malloc(sizeof(uint32_t) * (j->eoff[j->C] ? j->eoff[j->C] : 1));
+    for (size_t g = j->g0; g < j->g1; g++) {
+        size_t f0 = 4 * g;
+        int nf = (int)((j->F - f0) < 4 ? (j->F - f0) : 4), done = 0;
+        for (int v = 0; v < N; v++) {
+            uint32_t x = 0;
+            for (int k = 0; k < nf; k++) x = put(x, k, j->llr[(f0 + k) * (size_t)N + v]);
+            post[v] = x;
+        }
+        for (int k = nf; k < 4; k++) done |= 1 << k;
+        for (int it = 0; it < j->max_iters && done != 15; it++) {
+            uint32_t ov = 0;
+            for (int c = 0; c < j->C; c++) {
+                int d = j->deg[c];
+                int lp = table_param(j->variant, j->table, j->tits, j->tdeg, it, d);
+                ov |= check_word(d, j->pos + j->eoff[c], post, msg + j->eoff[c], 1, 0, it == 0,
+                                 j->variant, 0, lp);
+            }
+            for (int k = 0; k < 4; k++) {
+                if (done >> k & 1) continue;
+                if (ub(ov, k) == 0 || it + 1 == j->max_iters) {
+                    uint8_t *h = j->hard + (f0 + k) * (size_t)N;
+                    for (int v = 0; v < N; v++) h[v] = (uint8_t)(sb(post[v], k) > 0);
+                    if (j->iters) j->iters[f0 + k] = it + 1;
+                    done |= 1 << k;
+                }
+            }
+        }
+    }
+    free(post);
+    free(msg);
+}
+
+static void *fjob_run(void *p) { frames_range((const fjob_t *)p); return NULL; }
+
+int orc_decode_frames(int N, int C, const int *deg, const uint32_t *pos, int variant,
+                      const int8_t *llr, size_t F, int max_iters, const int8_t *table,
+                      int tits, int tdeg, uint8_t *hard, int *iters, int threads) {
+    size_t *eoff;
+    if (!llr || !hard || variant == ORC_OMS) return -1;
+    if (validate(N, C, deg, pos, variant, max_iters, table, tits, tdeg, &eoff)) return -1;
+    size_t G = (F + 3) / 4;
+    if (threads < 1) threads = 1;
+    if ((size_t)threads > G) threads = (int)(G ? G : 1);
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    fjob_t *jobs = (fjob_t *)malloc(sizeof(fjob_t) * (size_t)threads);
+    for (int t = 0; t < threads; t++) {
+        fjob_t j = {N, C, deg, pos, eoff, variant, llr, F, max_iters, table, tits, tdeg, hard, iters,
+                    G * (size_t)t / (size_t)threads, G * (size_t)(t + 1) / (size_t)threads};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, fjob_run, &jobs[t]);
     }
     for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
     free(th);
@@ -469,7 +545,7 @@ static void shadow_check(int d, const float *r, float par_val, int mode, float *
     }
     for (int j = 0; j < d; j++) {
         float mo = (j == i1) ? min2 : min1;
-        float mag = mode == 0 ? clampf(mo - par_val, 0.0f, 31.0f) : par_val * mo;
+        float mag = mode == 0 ? clampf(mo - par_val, 0.0f, 31.0f) : clampf(par_val * mo, 0.0f, 31.0f);
         int pos_others = par ^ (t[j] > 0.0f);
         mnew[j] = pos_others ? mag : -mag;
     }
@@ -569,8 +645,10 @@ static double shadow_run(int N, int C, const int *deg, const uint32_t *pos, int 
                         gmo = inside ? gmag : 0.0f;
                         grad[(size_t)it * dcols + col] -= inside ? gmag : 0.0f;
                     } else {
-                        gmo = gmag * pv;
-                        grad[(size_t)it * dcols + col] += gmag * mo;
+                        const float x = pv * mo;
+                        const int inside = x > 0.0f && x < 31.0f;
+                        gmo = inside ? gmag * pv : 0.0f;
+                        grad[(size_t)it * dcols + col] += inside ? gmag * mo : 0.0f;
                     }
                     /* mo is |t| of the edge holding min1, or min2 for that edge itself */
                     if (j == i1) {

@@ -27,7 +27,7 @@ enum {
     ORC_NMS = 3,     /* src/decoder_nms/cuda/CUDA_NMS_SIMD.cu:25-292          */
     ORC_2NMS = 4,    /* src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:25-292        */
     ORC_LOMS = 5,    /* learned-offset OMS: table[iteration][check degree]    */
-    ORC_LNMS = 6     /* learned-normalised MS: table[iteration][degree] / 8   */
+    ORC_LNMS = 6     /* learned-normalised MS: min(m * table / 8, 31)         */
 };
 
 /* ---- per-byte SIMD ops (semantics: src/decoder_ms/cuda/simd_functions.h:89-131) */
@@ -58,7 +58,8 @@ uint32_t orc_vsetgts4(uint32_t a, uint32_t b);
  *            (src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:164-166,237-244)
  * stopped  : [W] per-word stop flags (in/out), may be NULL when !early_term
  * table    : learned table [iter_base + it][degree] (LOMS: offset in LSB,
- *            LNMS: numerator over 8), row length tdeg, rows tits; else NULL
+ *            LNMS: numerator over 8, result capped at 31), row length tdeg,
+ *            rows tits; else NULL
  * iter_base: absolute index of this call's first iteration (learned tables)
  * iters_run: [W] out, iterations executed per word in this call (may be NULL)
  * returns 0, or -1 on bad arguments. */
@@ -74,6 +75,16 @@ int orc_decode_mt(int N, int C, const int *deg, const uint32_t *pos, int variant
                   int q1_split, int early_term, uint8_t *stopped,
                   const int8_t *table, int tits, int tdeg, int iter_base,
                   int *iters_run, int threads);
+
+/* Per-frame stop (the persistent frame-queue decoder's model): each frame is
+ * decoded alone (in its own byte lane) from iteration 0, with the stop test of
+ * CUDA_2NMS_SIMD.cu:237-244 applied to it after every iteration, the first
+ * included, up to max_iters.  llr / hard: frame-major [F][N]; iters[F] (may be
+ * NULL) = iterations run.  Variant OMS is refused: its first-iteration rule
+ * depends on the degree group (use OMS_V2 = OMS on a code with one group). */
+int orc_decode_frames(int N, int C, const int *deg, const uint32_t *pos, int variant,
+                      const int8_t *llr, size_t F, int max_iters, const int8_t *table,
+                      int tits, int tdeg, uint8_t *hard, int *iters, int threads);
 
 /* ---- layouts (src/transpose/GPU_Transpose_uint8.cu:9-130) --------------- */
 /* frame-major int8 in[(4w+k)*N + v]  ->  post[v*W + w] byte k               */
@@ -112,7 +123,7 @@ void orc_syndrome(const uint8_t *hard, int N, int C, const int *deg,
  *   t = clamp(p - m, -128, 127)   (first iteration: m = 0)
  *   mo_j = min of the OTHER |t| (127 cap), sign by parity of the others' t > 0
  *   mode 0 (offset): mag = clamp(mo - beta[it][d], 0, 31)
- *   mode 1 (scale):  mag = alpha[it][d] * mo
+ *   mode 1 (scale):  mag = clamp(alpha[it][d] * mo, 0, 31)
  *   m_j = +-mag, p = clamp(t + m_j, -128, 127)
  * loss per frame = mean over iterations and variables of softplus(p / 8)
  * (all-zero codeword: a positive value votes for bit 1).

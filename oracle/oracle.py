@@ -59,6 +59,9 @@ def lib() -> C.CDLL:
                                            C.POINTER(C.c_float), _i8_p, C.c_size_t, C.POINTER(C.c_float)]
         L.orc_shadow_forward.restype = C.c_double
         L.orc_shadow_forward.argtypes = L.orc_shadow_loss_grad.argtypes
+        L.orc_decode_frames.restype = C.c_int
+        L.orc_decode_frames.argtypes = [C.c_int, C.c_int, _i_p, _u32_p, C.c_int, _i8_p, C.c_size_t, C.c_int, _i8_p,
+                                        C.c_int, C.c_int, _u8_p, _i_p, C.c_int]
         L.orc_syndrome.restype = None
         L.orc_syndrome.argtypes = [_u8_p, C.c_int, C.c_int, _i_p, _u32_p, C.c_size_t, _u8_p]
         _lib = L
@@ -87,6 +90,26 @@ def deinterleave_hd(post: np.ndarray, n: int) -> np.ndarray:
     out = np.empty(4 * W * n, np.uint8)
     lib().orc_deinterleave_hd(_p(post, C.c_uint32), _p(out, C.c_uint8), n, W)
     return out
+
+
+def decode_frames(deg, pos, n_vars: int, variant: int, llr: np.ndarray, max_iters: int, table=None,
+                  threads: int = 1):
+    """Per-frame stop decode (the frame-queue decoder's model): returns
+    (hard [F][N] uint8, iterations [F] int32)."""
+    deg = np.ascontiguousarray(deg, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.uint32)
+    llr = np.ascontiguousarray(llr, dtype=np.int8).reshape(-1, n_vars)
+    F = llr.shape[0]
+    hard = np.empty((F, n_vars), np.uint8)
+    its = np.empty(F, np.int32)
+    t = None if table is None else np.ascontiguousarray(table, dtype=np.int8)
+    targs = (_p(t, C.c_int8), t.shape[0], t.shape[1]) if t is not None else (None, 0, 0)
+    rc = lib().orc_decode_frames(n_vars, deg.size, _p(deg, C.c_int), _p(pos, C.c_uint32), variant,
+                                 _p(llr, C.c_int8), F, max_iters, *targs, _p(hard, C.c_uint8), _p(its, C.c_int),
+                                 threads)
+    if rc != 0:
+        raise ValueError("oracle rejected the arguments")
+    return hard, its
 
 
 class OracleDecoder:

@@ -1,0 +1,112 @@
+"""GPU parity of the persistent frame-queue decoder (ldpc_pool_*) against the
+oracle's per-frame stop decode: decisions and iteration count of every frame
+bit-exact, for every variant, with slots refilled many times over (few
+workgroups, many frames), on codes that exercise each degree class, a frame
+length that is not a multiple of four, learned tables and the default grid."""
+import os
+
+import numpy as np
+import pytest
+
+import fast_ldpc_decoder_for_gpu_fixed_amd as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def mixed_llr(n, frames, seed, lo=1.0, hi=3.5, rate=0.5):
+    """All-zero codeword, BPSK bit0 -> -1, per-frame Eb/N0 in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    ebn0 = rng.uniform(lo, hi, size=(frames, 1))
+    sigma = np.sqrt(1.0 / (2 * rate * 10 ** (ebn0 / 10)))
+    y = -1.0 + sigma * rng.standard_normal((frames, n))
+    return np.clip(np.round(16.0 * y / sigma ** 2), -127, 127).astype(np.int8)
+
+
+def table_for(variant, iters=20, degs=33, seed=5):
+    if variant not in (L.LOMS, L.LNMS):
+        return None
+    rng = np.random.default_rng(seed)
+    lo, hi = (0, 3) if variant == L.LOMS else (5, 8)
+    return rng.integers(lo, hi + 1, size=(iters, degs)).astype(np.int8)
+
+
+def check_pool(code, variant, llr, max_it, workgroups=0, **tuning):
+    deg, pos = code.export()
+    table = table_for(variant)
+    pool = L.PoolDecoder(code, variant, max_it, table=table, **tuning)
+    hard, its = pool.decode(llr, workgroups=workgroups)
+    ref_h, ref_i = O.decode_frames(deg, pos, code.n_vars, L.OMS_V2 if variant == L.OMS else variant, llr, max_it,
+                                   table=table, threads=THREADS)
+    np.testing.assert_array_equal(its, ref_i, err_msg="iterations per frame")
+    np.testing.assert_array_equal(hard, ref_h, err_msg="decisions")
+    return its
+
+
+_CODES = {}
+
+
+def code(name):
+    if name not in _CODES:
+        _CODES[name] = {
+            "c36_1000": lambda: L.code_3_6(1000, seed=3),
+            "c36_1002": lambda: L.code_3_6(1002, seed=4),
+            "qc_4x24_z24": lambda: L.Code.qc(4, 24, 24, 0.9, seed=2),
+            "qc_12x24_z27": lambda: L.Code.qc(12, 24, 27, 0.9, seed=2),
+        }[name]()
+    return _CODES[name]
+
+
+@pytest.mark.parametrize("variant", [L.MS, L.OMS, L.OMS_V2, L.NMS, L.NMS2, L.LOMS, L.LNMS],
+                         ids=lambda v: L.VARIANT_NAMES[v])
+def test_pool_parity_refills(variant, gpu_available):
+    c = code("c36_1000")
+    llr = mixed_llr(c.n_vars, 2000, seed=variant)
+    its = check_pool(c, variant, llr, 20, workgroups=3)
+    assert (its < 20).any()
+
+
+@pytest.mark.parametrize("name", ["c36_1002", "qc_4x24_z24", "qc_12x24_z27"])
+@pytest.mark.parametrize("variant", [L.OMS_V2, L.NMS2, L.LOMS])
+def test_pool_parity_codes(name, variant, gpu_available):
+    c = code(name)
+    rate = 1.0 - c.n_checks / c.n_vars
+    llr = mixed_llr(c.n_vars, 1500, seed=7, lo=1.5, hi=4.0, rate=rate)
+    check_pool(c, variant, llr, 25, workgroups=5)
+
+
+@pytest.mark.parametrize("P,T", [(1, 64), (2, 128), (4, 256), (16, 256)])
+def test_pool_tunings_and_default_grid(P, T, gpu_available):
+    c = code("c36_1000")
+    llr = mixed_llr(c.n_vars, 1200, seed=P)
+    check_pool(c, L.OMS_V2, llr, 15, workgroups=0, words_per_block=P, threads=T)
+
+
+def test_pool_small_queue_and_reuse(gpu_available):
+    """Fewer frames than slots, then a second call on the same pool."""
+    c = code("c36_1000")
+    pool = L.PoolDecoder(c, L.MS, 10)
+    deg, pos = c.export()
+    for F, seed in ((3, 1), (777, 2)):
+        llr = mixed_llr(c.n_vars, F, seed=seed)
+        hard, its = pool.decode(llr)
+        rh, ri = O.decode_frames(deg, pos, c.n_vars, L.MS, llr, 10)
+        np.testing.assert_array_equal(its, ri)
+        np.testing.assert_array_equal(hard, rh)
+
+
+def test_pool_errors(gpu_available):
+    grouped = L.Code.qc(12, 24, 27, 0.9, seed=2)
+    if grouped.q1_split < grouped.n_checks:
+        with pytest.raises(L.LdpcError):
+            L.PoolDecoder(grouped, L.OMS, 10)
+    c = code("c36_1000")
+    with pytest.raises(L.LdpcError):
+        L.PoolDecoder(c, L.MS, 0)
+    with pytest.raises(L.LdpcError):
+        L.PoolDecoder(c, L.MS, 10, words_per_block=3)
+    p = L.PoolDecoder(c, L.LOMS, 10)
+    d = L.DeviceBuffer(c.n_vars * 4)
+    with pytest.raises(L.LdpcError):  # learned variant without a table
+        p.decode_device(d.ptr, 4, d.ptr)

@@ -49,7 +49,7 @@ def textbook(ts, variant, nocap=False, table_val=0):
         elif variant == O.LOMS:
             mag = min(max(m - table_val, 0), 31)
         elif variant == O.LNMS:
-            mag = (m * table_val) // 8
+            mag = min((m * table_val) // 8, 31)
         pos_others = sum(1 for k, x in enumerate(ts) if k != j and x > 0)
         msg = mag if pos_others % 2 == 1 else -mag
         msg = s8(msg)
