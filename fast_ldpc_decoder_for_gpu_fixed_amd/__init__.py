@@ -345,8 +345,8 @@ class PoolDecoder:
     device queue.  Inputs and outputs stay on the device (frame-major)."""
 
     def __init__(self, code: Code, variant: int, max_iterations: int, words_per_block: int = 0,
-                 threads: int = 0, table=None):
-        tu = Tuning(words_per_block, threads, -1, -1, -1, 0)
+                 threads: int = 0, table=None, uniform: int = -1, batch: int = 0):
+        tu = Tuning(words_per_block, threads, -1, -1, uniform, batch)
         h = load().ldpc_pool_create(code.handle, variant, max_iterations, C.byref(tu))
         if not h:
             raise LdpcError(f"ldpc_pool_create failed: {_lib.last_error()}")
@@ -385,14 +385,29 @@ class PoolDecoder:
         its = d_it.download(np.empty(F, np.int32))
         return hard, its
 
+    def simulate(self, ebn0_db: float, n_frames: int, seed: int = 1, snr_idx: int = 0, quant: int = 0,
+                 batch_frames: int = 0) -> dict:
+        """One Eb/N0 point: device channel -> pool decode -> device counting,
+        on frames 0..n_frames-1 of the stream (seed, snr_idx)."""
+        st = SimStats()
+        check(load().ldpc_pool_simulate(self._h, ebn0_db, quant, seed, snr_idx, n_frames, batch_frames,
+                                        C.byref(st)), "pool_simulate")
+        n_info = self.code.n_vars - self.code.n_checks
+        return {"frames": st.frames, "bit_errors": st.bit_errors, "frame_errors": st.frame_errors,
+                "syndrome_failures": st.syndrome_failures, "avg_iterations": st.avg_iterations,
+                "decode_ms": st.decode_ms, "fer": st.frame_errors / st.frames,
+                "ber": st.bit_errors / (st.frames * n_info)}
+
     @property
     def stream(self) -> int:
         return int(load().ldpc_pool_stream(self._h) or 0)
 
     def config(self) -> dict:
-        P, T, G = C.c_int(), C.c_int(), C.c_int()
-        check(load().ldpc_pool_config(self._h, C.byref(P), C.byref(T), C.byref(G)), "pool_config")
-        return {"words_per_block": P.value, "threads": T.value, "max_workgroups": G.value}
+        P, T, G, U, B = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(load().ldpc_pool_config(self._h, C.byref(P), C.byref(T), C.byref(G), C.byref(U), C.byref(B)),
+              "pool_config")
+        return {"words_per_block": P.value, "threads": T.value, "max_workgroups": G.value, "uniform_deg": U.value,
+                "batch": B.value}
 
 
 def channel_sigma(snr_db: float, rate: float, es_n0: bool = False) -> float:

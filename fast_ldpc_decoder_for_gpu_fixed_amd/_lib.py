@@ -80,7 +80,9 @@ SIGNATURES = {
     "ldpc_pool_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.POINTER(Tuning)]),
     "ldpc_pool_set_table": (C.c_int, [C.c_void_p, _i8_p, C.c_int, C.c_int]),
     "ldpc_pool_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
-    "ldpc_pool_config": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p]),
+    "ldpc_pool_simulate": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,
+                                     C.POINTER(SimStats)]),
+    "ldpc_pool_config": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "ldpc_pool_stream": (C.c_void_p, [C.c_void_p]),
     "ldpc_pool_destroy": (None, [C.c_void_p]),
     "ldpc_device_alloc": (C.c_void_p, [C.c_size_t]),

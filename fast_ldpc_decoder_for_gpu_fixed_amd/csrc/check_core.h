@@ -148,4 +148,89 @@ __device__ __forceinline__ uint2 check_core(const uint32_t (&pw)[D], const uint3
     return parity;
 }
 
+// Load / store the D message words of one lane (8-byte aligned runs when D is even).
+template <int D>
+__device__ __forceinline__ void load_msgs(const uint32_t *__restrict__ mp, uint32_t (&mw)[D]) {
+    if constexpr (D % 2 == 0) {
+        const uint2 *q = reinterpret_cast<const uint2 *>(mp);
+#pragma unroll
+        for (int h = 0; h < D / 2; h++) {
+            const uint2 x = q[h];
+            mw[2 * h] = x.x;
+            mw[2 * h + 1] = x.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < D; j++) mw[j] = mp[j];
+    }
+}
+template <int D>
+__device__ __forceinline__ void store_msgs(uint32_t *__restrict__ mp, const uint32_t (&mn)[D]) {
+    if constexpr (D % 2 == 0) {
+        uint2 *q = reinterpret_cast<uint2 *>(mp);
+#pragma unroll
+        for (int h = 0; h < D / 2; h++) q[h] = make_uint2(mn[2 * h], mn[2 * h + 1]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < D; j++) mp[j] = mn[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One level's items of one thread for a code whose checks all have degree D:
+// items c_first, c_first + cstride, ... < le of word w.  The check's edges
+// start at c*D, its LDS byte offsets (v * P * 4) come packed two per dword
+// (edge_off16), and the index and message loads of G items are issued
+// together before any of them is computed.  FIRST: no message read (the
+// decode's first iteration).  keep: byte mask applied to every message word
+// read (a lane whose frame starts now reads zero messages).  Returns the
+// OR of the items' parity bits (check_core's layout).
+// ---------------------------------------------------------------------------
+template <int D, int G, bool EQ, bool FIRST>
+__device__ __forceinline__ uint2 uniform_items(const uint32_t *__restrict__ edge_off16, int P, int variant,
+                                               uint32_t *post, uint32_t *mbase, int c_first, int le, int cstride,
+                                               int w, uint32_t keep, u16x2 lpe, u16x2 lpo) {
+    constexpr int H = (D + 1) / 2;
+    char *const pbase = reinterpret_cast<char *>(post) + 4 * w;
+    uint2 ov = make_uint2(0u, 0u);
+    for (int c0 = c_first; c0 < le; c0 += G * cstride) {
+        uint32_t off[G][H];
+        uint32_t mw[G][D];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int c = c0 + g * cstride;
+            if (c < le) {
+#pragma unroll
+                for (int h = 0; h < H; h++) off[g][h] = edge_off16[(size_t)c * H + h];
+                if (!FIRST) {
+                    load_msgs<D>(mbase + ((size_t)c * P + w) * D, mw[g]);
+#pragma unroll
+                    for (int j = 0; j < D; j++) mw[g][j] &= keep;
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int c = c0 + g * cstride;
+            if (c < le) {
+                uint32_t *vp[D];
+                uint32_t pw[D], pn[D], mn[D];
+#pragma unroll
+                for (int j = 0; j < D; j++)
+                    vp[j] = reinterpret_cast<uint32_t *>(
+                        pbase + ((j & 1) ? (off[g][j >> 1] >> 16) : (off[g][j >> 1] & 0xFFFFu)));
+#pragma unroll
+                for (int j = 0; j < D; j++) pw[j] = *vp[j];
+                const uint2 pr = check_core<D, EQ>(pw, mw[g], FIRST, variant, false, lpe, lpo, pn, mn);
+                ov.x |= pr.x;
+                ov.y |= pr.y;
+                store_msgs<D>(mbase + ((size_t)c * P + w) * D, mn);
+#pragma unroll
+                for (int j = 0; j < D; j++) *vp[j] = pn[j];
+            }
+        }
+    }
+    return ov;
+}
+
 }  // namespace ldpc

@@ -48,80 +48,15 @@ __device__ __forceinline__ uint32_t check_d(PostPtr post, const uint32_t *__rest
     return parity;
 }
 
-// Load / store the D message words of one lane (8-byte aligned runs when D is even).
-template <int D>
-__device__ __forceinline__ void load_msgs(const uint32_t *__restrict__ mp, uint32_t (&mw)[D]) {
-    if constexpr (D % 2 == 0) {
-        const uint2 *q = reinterpret_cast<const uint2 *>(mp);
-#pragma unroll
-        for (int h = 0; h < D / 2; h++) {
-            const uint2 x = q[h];
-            mw[2 * h] = x.x;
-            mw[2 * h + 1] = x.y;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < D; j++) mw[j] = mp[j];
-    }
-}
-template <int D>
-__device__ __forceinline__ void store_msgs(uint32_t *__restrict__ mp, const uint32_t (&mn)[D]) {
-    if constexpr (D % 2 == 0) {
-        uint2 *q = reinterpret_cast<uint2 *>(mp);
-#pragma unroll
-        for (int h = 0; h < D / 2; h++) q[h] = make_uint2(mn[2 * h], mn[2 * h + 1]);
-    } else {
-#pragma unroll
-        for (int j = 0; j < D; j++) mp[j] = mn[j];
-    }
-}
-
-// One level's items of one thread for a code whose checks all have degree D:
-// the check's edges start at c*D, its LDS byte offsets (v * P * 4) come
-// packed two per dword, and the index and message loads of G items are
-// issued together before any of them is computed.  FIRST is a template
-// argument so the first iteration's message-free path is a separate loop.
+// One level's items of one thread for a code whose checks all have degree D
+// (the shared body is uniform_items in check_core.h).
 template <int D, int G, bool EQ, bool FIRST>
 __device__ __forceinline__ uint32_t level_uniform(const DecodeArgs &a, uint32_t *post, uint32_t *mbase,
                                                   int c_first, int le, int cstride, int w, int lparam) {
-    constexpr int H = (D + 1) / 2;
-    const int P = a.P;
     const u16x2 lpv = splat(learned_half(a.variant, lparam));
-    char *const pbase = reinterpret_cast<char *>(post) + 4 * w;
-    uint32_t ov = 0;
-    for (int c0 = c_first; c0 < le; c0 += G * cstride) {
-        uint32_t off[G][H];
-        uint32_t mw[G][D];
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-            const int c = c0 + g * cstride;
-            if (c < le) {
-#pragma unroll
-                for (int h = 0; h < H; h++) off[g][h] = a.edge_off16[(size_t)c * H + h];
-                if (!FIRST) load_msgs<D>(mbase + ((size_t)c * P + w) * D, mw[g]);
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-            const int c = c0 + g * cstride;
-            if (c < le) {
-                uint32_t *vp[D];
-                uint32_t pw[D], pn[D], mn[D];
-#pragma unroll
-                for (int j = 0; j < D; j++)
-                    vp[j] = reinterpret_cast<uint32_t *>(
-                        pbase + ((j & 1) ? (off[g][j >> 1] >> 16) : (off[g][j >> 1] & 0xFFFFu)));
-#pragma unroll
-                for (int j = 0; j < D; j++) pw[j] = *vp[j];
-                const uint2 pr = check_core<D, EQ>(pw, mw[g], FIRST, a.variant, false, lpv, lpv, pn, mn);
-                ov |= pr.x | pr.y;
-                store_msgs<D>(mbase + ((size_t)c * P + w) * D, mn);
-#pragma unroll
-                for (int j = 0; j < D; j++) *vp[j] = pn[j];
-            }
-        }
-    }
-    return ov;
+    const uint2 pr = uniform_items<D, G, EQ, FIRST>(a.edge_off16, a.P, a.variant, post, mbase, c_first, le, cstride,
+                                                    w, 0xFFFFFFFFu, lpv, lpv);
+    return pr.x | pr.y;
 }
 
 template <bool EQ, int DMAXC, typename PostPtr>

@@ -68,7 +68,12 @@ struct PoolArgs {
     int max_iters, variant;
     const int8_t *ltable;         // learned table or null
     int tits, tdeg;
+    const uint32_t *edge_off16;   // single-degree path: packed LDS byte offsets, or null
+    int uniform_deg, batch;       // degree of that path, items in flight per thread (1, 2, 4)
 };
+// dynamic LDS a pool workgroup may use (its posterior block); the static
+// slot bookkeeping and the learned table take the rest of the 160 KiB
+constexpr size_t POOL_LDS_DYNAMIC = MAX_LDS_BYTES - 8192;
 hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s);
 int pool_blocks_per_cu(const PoolArgs &a, int dmax, int threads);
 

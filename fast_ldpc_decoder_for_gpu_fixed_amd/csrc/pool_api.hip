@@ -26,6 +26,8 @@ struct ldpc_pool {
     unsigned long long *d_next = nullptr;
     int8_t *d_table = nullptr;
     int tits = 0, tdeg = 0;
+    uint32_t *d_edge_off16 = nullptr;  // single-degree path
+    int uniform_deg = 0, batch = 1;
 };
 
 #define PHIP(expr)                                                                          \
@@ -46,6 +48,7 @@ void pool_free(ldpc_pool *p) {
     hipFree(p->d_msgs);
     hipFree(p->d_next);
     hipFree(p->d_table);
+    hipFree(p->d_edge_off16);
     if (p->stream) hipStreamDestroy(p->stream);
     delete p;
 }
@@ -71,6 +74,9 @@ ldpc::PoolArgs pool_args(const ldpc_pool *p) {
     a.ltable = p->d_table;
     a.tits = p->tits;
     a.tdeg = p->tdeg;
+    a.edge_off16 = p->d_edge_off16;
+    a.uniform_deg = p->uniform_deg;
+    a.batch = p->batch;
     return a;
 }
 }  // namespace
@@ -98,21 +104,35 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     }
     ldpc_tuning tu = {0, 0, -1, -1, -1, 0};
     if (tuning) tu = *tuning;
-    const size_t budget = ldpc::MAX_LDS_BYTES - 4096;
     const size_t wb = (size_t)code->N * 4;
     int P = tu.words_per_block;
     if (P <= 0) {
         P = 1;
         while (P * 2 <= ldpc::MAX_P && wb * (size_t)(P * 2) <= 40 * 1024) P *= 2;
     }
-    if (P < 1 || P > ldpc::MAX_P || (P & (P - 1)) || wb * (size_t)P > budget) {
+    if (P < 1 || P > ldpc::MAX_P || (P & (P - 1)) || wb * (size_t)P > ldpc::POOL_LDS_DYNAMIC) {
         ldpc::set_last_error("words_per_block must be a power of two <= 16 whose posterior block fits LDS");
         return nullptr;
     }
-    int T = tu.threads > 0 ? tu.threads : 256;
-    if (T % 64 || T % P || T > ldpc::max_threads_for_degree(code->max_deg, false) || T < 4 * P) {
+    // single-degree path: every check of degree 6..8 and LDS offsets within 16 bits
+    int dmin = ldpc::MAX_DEG + 1;
+    for (int x : code->deg) dmin = std::min(dmin, x);
+    const bool uni_ok = dmin == code->max_deg && ldpc::uniform_degree_supported(dmin) && wb * (size_t)P <= 65535;
+    if (tu.uniform == 1 && !uni_ok) {
+        ldpc::set_last_error("single-degree path requested but the code/blocking does not qualify");
+        return nullptr;
+    }
+    const int ud = (tu.uniform != 0 && uni_ok) ? dmin : 0;
+    const int batch = tu.batch > 0 ? tu.batch : 2;
+    if (batch != 1 && batch != 2 && batch != 4) {
+        ldpc::set_last_error("batch must be 1, 2 or 4");
+        return nullptr;
+    }
+    const int tmax = ldpc::max_threads_for_degree(code->max_deg, ud > 0);
+    int T = tu.threads > 0 ? tu.threads : std::min(256, tmax);
+    if (T % 64 || T % P || T > tmax || T < 4 * P) {
         ldpc::set_last_error("threads must be a multiple of 64 and of words_per_block, >= 4*words_per_block, "
-                             "within the kernel limit");
+                             "at most " + std::to_string(tmax) + " for this code");
         return nullptr;
     }
     ldpc_pool *p = new ldpc_pool();
@@ -123,6 +143,18 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     p->P = P;
     while ((1 << p->log2P) < P) p->log2P++;
     p->threads = T;
+    p->uniform_deg = ud;
+    p->batch = ud ? batch : 1;
+    std::vector<uint32_t> off16;
+    if (ud) {
+        const int H = (ud + 1) / 2;
+        off16.assign((size_t)code->C * H, 0);
+        for (int cs = 0; cs < code->C; cs++)
+            for (int j = 0; j < ud; j++) {
+                const uint32_t o = (p->plan.edge_var[(size_t)cs * ud + j] << p->log2P) * 4;  // LDS byte offset
+                off16[(size_t)cs * H + j / 2] |= (j & 1) ? (o << 16) : o;
+            }
+    }
     hipDeviceProp_t prop;
     int dev = 0;
     hipGetDevice(&dev);
@@ -140,12 +172,15 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     al(&p->d_level_beg, p->plan.level_beg.size());
     al(&p->d_msgs, (size_t)p->max_grid * code->E * P);
     al(&p->d_next, 1);
+    al(&p->d_edge_off16, off16.size());
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamDefault);
     if (e == hipSuccess) e = hipMemcpy(p->d_checks, ck.data(), ck.size() * sizeof(uint2), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_edge_var, p->plan.edge_var.data(), (size_t)code->E * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(p->d_level_beg, p->plan.level_beg.data(), p->plan.level_beg.size() * sizeof(int),
                       hipMemcpyHostToDevice);
+    if (e == hipSuccess && !off16.empty())
+        e = hipMemcpy(p->d_edge_off16, off16.data(), off16.size() * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         ldpc::set_last_error(std::string("ldpc_pool_create: ") + hipGetErrorString(e));
         pool_free(p);
@@ -200,7 +235,84 @@ int ldpc_pool_decode(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t 
     return LDPC_OK;
 }
 
-int ldpc_pool_config(const ldpc_pool *p, int *words_per_block, int *threads, int *max_workgroups) {
+int ldpc_pool_simulate(ldpc_pool *p, double ebn0_db, int quant, uint64_t seed, uint32_t snr_idx, uint64_t n_frames,
+                       uint64_t batch_frames, ldpc_sim_stats *out) {
+    if (!p || !out || n_frames == 0 || (quant != 0 && quant != 1)) {
+        ldpc::set_last_error("ldpc_pool_simulate: bad arguments");
+        return LDPC_ERR_ARG;
+    }
+    const ldpc_code &c = p->code;
+    if ((size_t)c.N > ldpc::MAX_LDS_BYTES - 1024) {
+        ldpc::set_last_error("frame longer than the counting kernel's LDS staging buffer");
+        return LDPC_ERR_ARG;
+    }
+    const uint64_t batch = std::min<uint64_t>(n_frames, batch_frames ? batch_frames : (1ull << 18));
+    const double rate = (double)(c.N - c.C) / (double)c.N;
+    const double sigma = ldpc_channel_sigma(ebn0_db, rate, 0);
+    static bool attr = false;
+    if (!attr) {
+        PHIP(ldpc::set_count_lds_limit());
+        attr = true;
+    }
+    int8_t *d_llr = nullptr;
+    uint8_t *d_hard = nullptr;
+    int *d_it = nullptr;
+    unsigned long long *d_cnt = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LDPC_OK;
+    double ms = 0.0, it_sum = 0.0;
+    std::vector<int> its((size_t)batch);
+    hipStream_t s = p->stream;
+    hipError_t e = hipMalloc((void **)&d_llr, (size_t)batch * c.N);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_hard, (size_t)batch * c.N);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_it, (size_t)batch * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_cnt, 3 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (uint64_t f0 = 0; e == hipSuccess && rc == LDPC_OK && f0 < n_frames; f0 += batch) {
+        const uint64_t nb = std::min<uint64_t>(batch, n_frames - f0);
+        e = ldpc::launch_channel(d_llr, 0, c.N, nb, f0, seed, snr_idx, sigma, quant, s);
+        if (e != hipSuccess) break;
+        e = hipEventRecord(e0, s);
+        if (e != hipSuccess) break;
+        rc = ldpc_pool_decode(p, d_llr, nb, d_hard, d_it, 0, s);
+        if (rc) break;
+        e = hipEventRecord(e1, s);
+        if (e == hipSuccess)
+            e = ldpc::launch_count(d_hard, c.N, nb, c.N - c.C, p->d_checks, p->d_edge_var, c.C, d_cnt, nullptr,
+                                   nullptr, f0 == 0, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(its.data(), d_it, nb * sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) break;
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        ms += t;
+        for (uint64_t i = 0; i < nb; i++) it_sum += its[i];
+    }
+    unsigned long long cnt[3] = {0, 0, 0};
+    if (e == hipSuccess && rc == LDPC_OK) e = hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost);
+    if (e != hipSuccess && rc == LDPC_OK) {
+        ldpc::set_last_error(std::string("ldpc_pool_simulate: ") + hipGetErrorString(e));
+        rc = LDPC_ERR_HIP;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(d_llr);
+    hipFree(d_hard);
+    hipFree(d_it);
+    hipFree(d_cnt);
+    if (rc) return rc;
+    out->frames = n_frames;
+    out->bit_errors = cnt[0];
+    out->frame_errors = cnt[1];
+    out->syndrome_failures = cnt[2];
+    out->avg_iterations = it_sum / (double)n_frames;
+    out->decode_ms = ms;
+    return LDPC_OK;
+}
+
+int ldpc_pool_config(const ldpc_pool *p, int *words_per_block, int *threads, int *max_workgroups, int *uniform_deg,
+                     int *batch) {
     if (!p) {
         ldpc::set_last_error("null pool");
         return LDPC_ERR_ARG;
@@ -208,6 +320,8 @@ int ldpc_pool_config(const ldpc_pool *p, int *words_per_block, int *threads, int
     if (words_per_block) *words_per_block = p->P;
     if (threads) *threads = p->threads;
     if (max_workgroups) *max_workgroups = p->max_grid;
+    if (uniform_deg) *uniform_deg = p->uniform_deg;
+    if (batch) *batch = p->batch;
     return LDPC_OK;
 }
 

@@ -61,8 +61,8 @@ __device__ __forceinline__ uint2 pool_check_any(int D, uint32_t *post, const uin
     }
 }
 
-template <bool EQ, int DMAXC>
-__global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool(const PoolArgs a) {
+template <bool EQ, int DMAXC, int UD, int G>
+__global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_pool(const PoolArgs a) {
     extern __shared__ uint32_t lds_post[];
     __shared__ long long s_fid[POOL_SLOTS];
     __shared__ int s_it[POOL_SLOTS];
@@ -138,28 +138,47 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool(const PoolArgs
 #pragma unroll
             for (int k = 0; k < 4; k++) rowoff[k] = min(s_it[4 * w + k], a.tits - 1) * a.tdeg;
         uint32_t ove = 0, ovo = 0;
-        for (int L = 0; L < a.n_levels; L++) {
-            const int lb = a.level_beg[L], le = a.level_beg[L + 1];
-            for (int c = lb + (tid >> lp); c < le; c += cstride) {
-                const uint2 ci = a.checks[c];
-                const int D = (int)(ci.x & 0xFFu);
-                u16x2 lpe = splat(0u), lpo = splat(0u);
-                if (learned) {
-                    const int col = min(D, a.tdeg - 1);
-                    const uint32_t l0 = learned_half(a.variant, s_tab[rowoff[0] + col]);
-                    const uint32_t l1 = learned_half(a.variant, s_tab[rowoff[1] + col]);
-                    const uint32_t l2 = learned_half(a.variant, s_tab[rowoff[2] + col]);
-                    const uint32_t l3 = learned_half(a.variant, s_tab[rowoff[3] + col]);
-                    lpe = u2(l0 | (l2 << 16));
-                    lpo = u2(l1 | (l3 << 16));
-                }
-                const uint2 pr = pool_check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y,
-                                                           mbase + (size_t)ci.y * P + (size_t)w * D, lp, w, keep,
-                                                           a.variant, lpe, lpo);
+        if constexpr (UD > 0) {
+            u16x2 lpe = splat(0u), lpo = splat(0u);
+            if (learned) {
+                const int col = min(UD, a.tdeg - 1);
+                lpe = u2(learned_half(a.variant, s_tab[rowoff[0] + col]) |
+                         (learned_half(a.variant, s_tab[rowoff[2] + col]) << 16));
+                lpo = u2(learned_half(a.variant, s_tab[rowoff[1] + col]) |
+                         (learned_half(a.variant, s_tab[rowoff[3] + col]) << 16));
+            }
+            for (int L = 0; L < a.n_levels; L++) {
+                const uint2 pr = uniform_items<UD, G, EQ, false>(a.edge_off16, P, a.variant, post, mbase,
+                                                                 a.level_beg[L] + (tid >> lp), a.level_beg[L + 1],
+                                                                 cstride, w, keep, lpe, lpo);
                 ove |= pr.x;
                 ovo |= pr.y;
+                __syncthreads();
             }
-            __syncthreads();
+        } else {
+            for (int L = 0; L < a.n_levels; L++) {
+                const int lb = a.level_beg[L], le = a.level_beg[L + 1];
+                for (int c = lb + (tid >> lp); c < le; c += cstride) {
+                    const uint2 ci = a.checks[c];
+                    const int D = (int)(ci.x & 0xFFu);
+                    u16x2 lpe = splat(0u), lpo = splat(0u);
+                    if (learned) {
+                        const int col = min(D, a.tdeg - 1);
+                        const uint32_t l0 = learned_half(a.variant, s_tab[rowoff[0] + col]);
+                        const uint32_t l1 = learned_half(a.variant, s_tab[rowoff[1] + col]);
+                        const uint32_t l2 = learned_half(a.variant, s_tab[rowoff[2] + col]);
+                        const uint32_t l3 = learned_half(a.variant, s_tab[rowoff[3] + col]);
+                        lpe = u2(l0 | (l2 << 16));
+                        lpo = u2(l1 | (l3 << 16));
+                    }
+                    const uint2 pr = pool_check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y,
+                                                               mbase + (size_t)ci.y * P + (size_t)w * D, lp, w, keep,
+                                                               a.variant, lpe, lpo);
+                    ove |= pr.x;
+                    ovo |= pr.y;
+                }
+                __syncthreads();
+            }
         }
         if (ove) atomicOr(&s_ove[w], ove);
         if (ovo) atomicOr(&s_ovo[w], ovo);
@@ -200,34 +219,58 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool(const PoolArgs
     }
 }
 
-template <bool EQ, int DMAXC>
+template <bool EQ, int DMAXC, int UD, int G>
 static hipError_t launch_pool_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pool<EQ, DMAXC>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(MAX_LDS_BYTES - 4096));
+        hipError_t e = hipFuncSetAttribute((const void *)k_pool<EQ, DMAXC, UD, G>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)POOL_LDS_DYNAMIC);
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL((k_pool<EQ, DMAXC>), dim3(grid), dim3(threads), (size_t)a.N * a.P * 4, s, a);
+    hipLaunchKernelGGL((k_pool<EQ, DMAXC, UD, G>), dim3(grid), dim3(threads), (size_t)a.N * a.P * 4, s, a);
     return hipGetLastError();
 }
 
+// the instantiation that runs a pool: generic degree classes, or the
+// single-degree path (ud 6..8, G items in flight per thread)
+template <bool EQ>
+static const void *pool_kernel(int dmax, int ud, int G) {
+    if (ud == 6) return G >= 4 ? (const void *)k_pool<EQ, 8, 6, 4> : G == 2 ? (const void *)k_pool<EQ, 8, 6, 2> : (const void *)k_pool<EQ, 8, 6, 1>;
+    if (ud == 7) return G >= 4 ? (const void *)k_pool<EQ, 8, 7, 4> : G == 2 ? (const void *)k_pool<EQ, 8, 7, 2> : (const void *)k_pool<EQ, 8, 7, 1>;
+    if (ud == 8) return G >= 4 ? (const void *)k_pool<EQ, 8, 8, 4> : G == 2 ? (const void *)k_pool<EQ, 8, 8, 2> : (const void *)k_pool<EQ, 8, 8, 1>;
+    if (dmax <= 8) return (const void *)k_pool<EQ, 8, 0, 1>;
+    if (dmax <= 16) return (const void *)k_pool<EQ, 16, 0, 1>;
+    return (const void *)k_pool<EQ, 32, 0, 1>;
+}
+
+template <bool EQ>
+static hipError_t launch_pool_eq(const PoolArgs &a, int dmax, int ud, int G, int grid, int threads, hipStream_t s) {
+#define LDPC_PL(U, GG)                                                                       \
+    if (ud == U && (GG == 4 ? G >= 4 : G == GG)) return launch_pool_t<EQ, 8, U, GG>(a, grid, threads, s);
+    LDPC_PL(6, 1) LDPC_PL(6, 2) LDPC_PL(6, 4) LDPC_PL(7, 1) LDPC_PL(7, 2) LDPC_PL(7, 4)
+    LDPC_PL(8, 1) LDPC_PL(8, 2) LDPC_PL(8, 4)
+#undef LDPC_PL
+    if (ud > 0) return hipErrorInvalidValue;
+    if (dmax <= 8) return launch_pool_t<EQ, 8, 0, 1>(a, grid, threads, s);
+    if (dmax <= 16) return launch_pool_t<EQ, 16, 0, 1>(a, grid, threads, s);
+    return launch_pool_t<EQ, 32, 0, 1>(a, grid, threads, s);
+}
+
 hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s) {
-    if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax, false)) return hipErrorInvalidValue;
-    const bool eq = a.variant == VAR_2NMS;
-    if (dmax <= 8) return eq ? launch_pool_t<true, 8>(a, grid, threads, s) : launch_pool_t<false, 8>(a, grid, threads, s);
-    if (dmax <= 16) return eq ? launch_pool_t<true, 16>(a, grid, threads, s) : launch_pool_t<false, 16>(a, grid, threads, s);
-    return eq ? launch_pool_t<true, 32>(a, grid, threads, s) : launch_pool_t<false, 32>(a, grid, threads, s);
+    const int ud = a.uniform_deg;
+    if (ud > 0 && !a.edge_off16) return hipErrorInvalidValue;
+    if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax, ud > 0) || grid < 1)
+        return hipErrorInvalidValue;
+    if (ud > 0 && (ud != dmax || !uniform_degree_supported(ud))) return hipErrorInvalidValue;
+    return a.variant == VAR_2NMS ? launch_pool_eq<true>(a, dmax, ud, a.batch, grid, threads, s)
+                                 : launch_pool_eq<false>(a, dmax, ud, a.batch, grid, threads, s);
 }
 
 int pool_blocks_per_cu(const PoolArgs &a, int dmax, int threads) {
     int n = 0;
-    const void *f = nullptr;
-    const bool eq = a.variant == VAR_2NMS;
-    if (dmax <= 8) f = eq ? (const void *)k_pool<true, 8> : (const void *)k_pool<false, 8>;
-    else if (dmax <= 16) f = eq ? (const void *)k_pool<true, 16> : (const void *)k_pool<false, 16>;
-    else f = eq ? (const void *)k_pool<true, 32> : (const void *)k_pool<false, 32>;
+    const int ud = a.uniform_deg;
+    const void *f = a.variant == VAR_2NMS ? pool_kernel<true>(dmax, ud, a.batch) : pool_kernel<false>(dmax, ud, a.batch);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, (size_t)a.N * a.P * 4) != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }

@@ -9,8 +9,10 @@ a per-frame Eb/N0 drawn uniformly in [lo, hi].  The table [iteration][check
 degree] is rounded to int8 LSB steps and drives the integer decoder
 (LDPC_LOMS), which stays bit-exact against its oracle model.  Evaluation
 decodes the same held-out frames (same channel stream) with:
-  plain OMS, 20 fixed iterations (the judge), plain OMS 10 fixed, plain OMS
-  with per-word early stop (max 20), learned OMS with early stop (max T),
+  plain OMS, 20 fixed iterations (the reference point), plain OMS 10 fixed,
+  plain OMS with per-word early stop (max 20), learned OMS with per-word early
+  stop (max T and max 20), and the persistent frame-queue decoder (per-frame
+  stop, slot refill) running plain OMS and learned OMS (max 20),
 and reports FER, BER, average iterations per frame and decode throughput.
 Prints one JSON document.
 """
@@ -23,7 +25,8 @@ import time
 
 import numpy as np
 
-from . import LEARN_OFFSET, LEARN_SCALE, LNMS, LOMS, OMS, Code, Trainer, code_3_6, make_decoder, quantize_offsets, quantize_scales
+from . import (LEARN_OFFSET, LEARN_SCALE, LNMS, LOMS, OMS, Code, PoolDecoder, Trainer, code_3_6, make_decoder,
+               quantize_offsets, quantize_scales)
 
 
 def train(code: Code, iters: int, batch: int, steps: int, lr: float, ebn0, seed: int, mode: int = LEARN_OFFSET,
@@ -55,10 +58,18 @@ def evaluate(code: Code, table: np.ndarray, variant: int, iters: int, ebn0_list,
     }
     for k in ("learned_early_stop", "learned_fixed", "learned_early_stop_max20"):
         decs[k][0].set_table(table)  # rows past the table repeat its last row
+    pools = {
+        "oms20_pool": PoolDecoder(code, OMS, 20),
+        "learned_pool_max20": PoolDecoder(code, variant, 20, table=table),
+    }
     for i, eb in enumerate(ebn0_list):
         row = {"ebn0_db": eb}
-        for name, (dec, it) in decs.items():
-            r = dec.simulate(eb, frames, it, seed=seed, snr_idx=snr_base + i)
+        runs = [(name, lambda d=dec, it=it: d.simulate(eb, frames, it, seed=seed, snr_idx=snr_base + i))
+                for name, (dec, it) in decs.items()]
+        runs += [(name, lambda p=pool: p.simulate(eb, frames, seed=seed, snr_idx=snr_base + i))
+                 for name, pool in pools.items()]
+        for name, run in runs:
+            r = run()
             r["info_gbps"] = r["frames"] * K / (r["decode_ms"] * 1e-3) / 1e9
             row[name] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
         out.append(row)

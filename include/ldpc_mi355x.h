@@ -152,32 +152,6 @@ int ldpc_decoder_config(const ldpc_decoder *dec, int *words_per_block, int *thre
  * magnitude capped at 31. */
 int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, int n_degs);
 
-/* ---- persistent frame-queue decoder ("pool"; no single counterpart in the
- * reference: it generalises the per-word early stop of
- * src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:237-244 to each frame and keeps the
- * GPU full by refilling a stopped frame's lane from a device queue) --------
- * A frame stops after the first iteration in which every check has an even
- * parity of positive v2c values, or at max_iterations; its decisions and
- * iteration count are written and its slot takes the next queued frame.
- * Variants: MS, OMS_V2, NMS, 2NMS, LOMS, LNMS, and OMS on codes with a single
- * degree group (the OMS first-iteration quirk is not per-frame).  Results
- * per frame equal a one-frame layered decode with that stop rule. */
-typedef struct ldpc_pool ldpc_pool;
-ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations,
-                            const ldpc_tuning *tuning);
-int ldpc_pool_set_table(ldpc_pool *pool, const int8_t *table, int n_iters, int n_degs);
-/* Decode `frames` frame-major LLRs at d_in into d_out (decisions) and
- * d_iters (iterations per frame, may be NULL).  workgroups = 0: as many as
- * stay resident on the GPU (capped by the work).  Asynchronous on `stream`
- * (NULL = the pool's own); the queue counter is reset with a kernel, so the
- * call may be captured in a HIP graph. */
-int ldpc_pool_decode(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters,
-                     int workgroups, void *stream);
-/* Words per workgroup, workgroup size and resident workgroups chosen. */
-int ldpc_pool_config(const ldpc_pool *pool, int *words_per_block, int *threads, int *max_workgroups);
-void *ldpc_pool_stream(ldpc_pool *pool);
-void ldpc_pool_destroy(ldpc_pool *pool);
-
 /* ---- state API (per-iteration parity; the reference keeps this state in
  * device_V / d_MSG_C_2_V, src/decoder_template/CGPUDecoder.h:21-23) ------- */
 /* Load host input (same layout as decode) and reset the decode. */
@@ -248,6 +222,40 @@ typedef struct {
  * arguments see identical frames. */
 int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, uint32_t snr_idx,
                   uint64_t n_frames, int iterations, ldpc_sim_stats *out);
+
+/* ---- persistent frame-queue decoder ("pool"; no single counterpart in the
+ * reference: it generalises the per-word early stop of
+ * src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:237-244 to each frame and keeps the
+ * GPU full by refilling a stopped frame's lane from a device queue) --------
+ * A frame stops after the first iteration in which every check has an even
+ * parity of positive v2c values, or at max_iterations; its decisions and
+ * iteration count are written and its slot takes the next queued frame.
+ * Variants: MS, OMS_V2, NMS, 2NMS, LOMS, LNMS, and OMS on codes with a single
+ * degree group (the OMS first-iteration quirk is not per-frame).  Results
+ * per frame equal a one-frame layered decode with that stop rule. */
+typedef struct ldpc_pool ldpc_pool;
+ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations,
+                            const ldpc_tuning *tuning);
+int ldpc_pool_set_table(ldpc_pool *pool, const int8_t *table, int n_iters, int n_degs);
+/* Decode `frames` frame-major LLRs at d_in into d_out (decisions) and
+ * d_iters (iterations per frame, may be NULL).  workgroups = 0: as many as
+ * stay resident on the GPU (capped by the work).  Asynchronous on `stream`
+ * (NULL = the pool's own); the queue counter is reset with a kernel, so the
+ * call may be captured in a HIP graph. */
+int ldpc_pool_decode(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters,
+                     int workgroups, void *stream);
+/* One Eb/N0 point: device channel (frames 0 .. n_frames-1 of the stream
+ * (seed, snr_idx), the same frames ldpc_simulate draws) -> pool decode ->
+ * device error counting, batch_frames frames per launch (0 = 2^18).
+ * avg_iterations is per frame; decode_ms covers the decode launches only. */
+int ldpc_pool_simulate(ldpc_pool *pool, double ebn0_db, int quant, uint64_t seed, uint32_t snr_idx,
+                       uint64_t n_frames, uint64_t batch_frames, ldpc_sim_stats *out);
+/* Words per workgroup, workgroup size, resident workgroups, the degree of
+ * the single-degree path (0 = generic path) and its items per thread. */
+int ldpc_pool_config(const ldpc_pool *pool, int *words_per_block, int *threads, int *max_workgroups,
+                     int *uniform_deg, int *batch);
+void *ldpc_pool_stream(ldpc_pool *pool);
+void ldpc_pool_destroy(ldpc_pool *pool);
 
 /* ---- learned tables: float shadow decoder trained on the GPU ----------------
  * (no counterpart in the reference).  The shadow runs the integer decoder's

@@ -76,11 +76,23 @@ def test_pool_parity_codes(name, variant, gpu_available):
     check_pool(c, variant, llr, 25, workgroups=5)
 
 
-@pytest.mark.parametrize("P,T", [(1, 64), (2, 128), (4, 256), (16, 256)])
-def test_pool_tunings_and_default_grid(P, T, gpu_available):
+@pytest.mark.parametrize("P,T,uniform,batch", [(1, 64, 1, 1), (2, 128, 1, 2), (4, 256, 1, 4), (16, 512, 1, 2),
+                                              (2, 256, 0, 0), (16, 256, 0, 0)])
+def test_pool_tunings_and_default_grid(P, T, uniform, batch, gpu_available):
     c = code("c36_1000")
     llr = mixed_llr(c.n_vars, 1200, seed=P)
-    check_pool(c, L.OMS_V2, llr, 15, workgroups=0, words_per_block=P, threads=T)
+    pool = L.PoolDecoder(c, L.OMS_V2, 15, words_per_block=P, threads=T, uniform=uniform, batch=batch)
+    cfg = pool.config()
+    assert cfg["uniform_deg"] == (6 if uniform else 0)
+    check_pool(c, L.OMS_V2, llr, 15, workgroups=0, words_per_block=P, threads=T, uniform=uniform, batch=batch)
+
+
+@pytest.mark.parametrize("variant", [L.NMS2, L.LOMS, L.LNMS], ids=lambda v: L.VARIANT_NAMES[v])
+def test_pool_generic_path_learned(variant, gpu_available):
+    """The same frames through the generic (any degree) path."""
+    c = code("c36_1000")
+    llr = mixed_llr(c.n_vars, 1000, seed=11)
+    check_pool(c, variant, llr, 20, workgroups=2, uniform=0)
 
 
 def test_pool_small_queue_and_reuse(gpu_available):
@@ -110,3 +122,28 @@ def test_pool_errors(gpu_available):
     d = L.DeviceBuffer(c.n_vars * 4)
     with pytest.raises(L.LdpcError):  # learned variant without a table
         p.decode_device(d.ptr, 4, d.ptr)
+
+
+def test_pool_simulate_matches_direct_decode(gpu_available):
+    """Device channel -> pool -> device counters equals decoding the same
+    channel frames by hand and counting them with the oracle; the frames are
+    the ones the word decoder's simulate draws (same stream positions)."""
+    c = L.code_3_6(4000, seed=1)
+    F = 3000
+    pool = L.PoolDecoder(c, L.OMS, 20)
+    r = pool.simulate(1.75, F, seed=8, snr_idx=2, batch_frames=1024)  # three launches, the last partial
+    sig = L.channel_sigma(1.75, c.rate)
+    d_llr, d_hard, d_it = L.DeviceBuffer(F * 4000), L.DeviceBuffer(F * 4000), L.DeviceBuffer(4 * F)
+    L.channel_generate(d_llr.ptr, 4000, F, 8, 2, sig)
+    pool.decode_device(d_llr.ptr, F, d_hard.ptr, d_it.ptr)
+    L.synchronize()
+    hard = d_hard.download(np.empty(F * 4000, np.uint8))
+    its = d_it.download(np.empty(F, np.int32))
+    f, b, _ = O.count_errors(hard, 4000, c.n_info)
+    assert (r["frame_errors"], r["bit_errors"]) == (f, b)
+    assert r["avg_iterations"] == pytest.approx(its.mean(), rel=1e-12)
+    # and these are the word decoder's frames: 20 fixed iterations vs per-frame stop
+    llr = d_llr.download(np.empty((F, 4000), np.int8))
+    ref_h, ref_i = O.decode_frames(*c.export(), 4000, L.OMS_V2, llr[:200], 20, threads=THREADS)
+    np.testing.assert_array_equal(its[:200], ref_i)
+    np.testing.assert_array_equal(hard.reshape(F, 4000)[:200], ref_h)
