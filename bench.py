@@ -1,10 +1,21 @@
 """Benchmark of the decoding hot path on one MI355X.
 
-One step = one decode of the whole resident batch: 2^16 frames of the
-(3,6) N=4000 code, plain OMS (beta = 1 LSB), 20 fixed iterations, no early
-stop, LLRs already in HBM.  Prints one JSON line (see README of the task
-contract): info Gbit/s, the dominant kernel's HBM roofline and the CPU
-baseline (the oracle's serial restatement, timed on the host cores).
+Default (the headline metric, info Gbit/s at matched FER, learned vs plain OMS):
+the (3,6) N=4000 code, 2^20 frames at Eb/N0 = 2.0 dB from the device channel,
+resident in HBM.  Two decoders run the same frames:
+  * plain  -- the drop-in decoder, plain OMS (beta = 1 LSB), 20 fixed iterations;
+  * learned -- the frame-queue decoder (persistent workgroups, per-frame stop,
+    slot refill) with the committed learned-offset table, at most 20 iterations.
+One step = one decode of the whole batch by the learned decoder; `value` is its
+info throughput.  The plain decoder is timed the same way beside it, and both
+FERs come from the device error counters on the same frames.
+
+--mode plain: the drop-in decoder alone on 2^16 frames, 20 fixed iterations
+(the plain-OMS HBM roofline line).
+
+Prints one JSON line: info Gbit/s, the dominant kernel's roofline (algorithmic
+bytes / kernel time from HIP events on the kernel's stream) and the CPU
+baseline (the oracle's restatement of the reference path on the host cores).
 """
 from __future__ import annotations
 
@@ -21,6 +32,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+TABLE = ROOT / "fast_ldpc_decoder_for_gpu_fixed_amd" / "tables" / "loms_3_6_n4000.json"
 
 
 def parse():
@@ -28,52 +40,218 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variant", default="OMS")
+    ap.add_argument("--mode", choices=["matched", "plain"], default="matched")
+    ap.add_argument("--variant", default="OMS", help="plain mode: MS, OMS, OMS_V2, NMS, 2NMS")
     ap.add_argument("--iters", type=int, default=20)
-    ap.add_argument("--frames", type=int, default=1 << 16)
+    ap.add_argument("--frames", type=int, default=0, help="default: 2^20 (matched), 2^16 (plain)")
     ap.add_argument("--ebn0", type=float, default=2.0)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--table", default=str(TABLE))
     ap.add_argument("--words-per-block", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--cpu-baseline-words", type=int, default=1024)
+    ap.add_argument("--cpu-baseline-words", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true", help="matched mode: time the learned decoder only")
     ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "hbm_traffic.json"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (HBM counter calibration)")
     return ap.parse_args()
 
 
-def channel_llr(n, frames, ebn0, rate=0.5, seed=1):
-    rng = np.random.default_rng(seed)
-    sigma = np.sqrt(1.0 / (2 * rate * 10 ** (ebn0 / 10)))
-    y = -1.0 + sigma * rng.standard_normal(frames * n, dtype=np.float32)
-    return np.clip(np.round(16.0 * y / sigma ** 2), -127, 127).astype(np.int8)
+def traffic_for(path, key):
+    """HBM bytes per launch measured by rocprofv3 counters for this workload."""
+    try:
+        t = json.loads(Path(path).read_text())
+        return t.get("workloads", {}).get(key, {}).get("bytes_per_launch")
+    except (ValueError, OSError):
+        return None
 
 
-def cpu_baseline(code, llr, n_words, iters, variant):
-    """The oracle (scalar C restatement of the reference kernel) on a bounded
-    sample of the same workload, all host cores (capped at 16)."""
+def cpu_baseline(code, llr, n_words, iters):
+    """The oracle (scalar C restatement of the reference kernel, plain OMS,
+    fixed iterations) on a bounded sample of the same frames, all host cores
+    (capped at 16)."""
     from oracle import oracle as O
     deg, pos = code.export()
     N = code.n_vars
     threads = min(16, os.cpu_count() or 1)
     sub = llr[: 4 * n_words * N]
-    orc = O.OracleDecoder(deg, pos, N, n_words, variant, q1_split=code.q1_split, early_term=0)
+    orc = O.OracleDecoder(deg, pos, N, n_words, O.OMS, q1_split=code.q1_split, early_term=0)
     orc.load(sub)
     t0 = time.perf_counter()
     orc.iterate(iters, threads=threads)
     dt = time.perf_counter() - t0
     frames = 4 * n_words
-    return {"value": frames * code.n_info / dt / 1e9, "unit": "info Gbit/s", "cores": threads,
+    return {"value": round(frames * code.n_info / dt / 1e9, 4), "unit": "info Gbit/s", "cores": threads,
             "kind": "port",
-            "sample": f"{frames} frames ({n_words} words) of the same code/input, {iters} iterations, "
-                      f"{dt:.2f} s on {threads} host threads"}
+            "sample": f"{frames} frames ({n_words} words) of the same code and input, plain OMS, {iters} fixed "
+                      f"iterations, {dt:.2f} s on {threads} host threads"}
+
+
+def timed(L, stream, steps, warmup, fn):
+    """ms per step of fn() over `steps` calls, HIP events on `stream`."""
+    for _ in range(warmup):
+        fn()
+    L.synchronize()
+    timer = L.CTimer(stream=stream)
+    timer.start()
+    for _ in range(steps):
+        fn()
+    timer.stop()
+    ms = timer.get_time_ms()
+    L.synchronize()
+    return ms / steps
+
+
+def counts(L, dec, d_hard, frames, n_info):
+    d_cnt = L.DeviceBuffer(24)
+    dec.count_errors(d_hard, frames, n_info, d_cnt.ptr, stream=dec.stream)
+    L.synchronize()
+    c = d_cnt.download(np.zeros(3, np.uint64))
+    return {"frame_errors": int(c[1]), "bit_errors": int(c[0]), "fer": float(c[1]) / frames,
+            "ber": float(c[0]) / (frames * n_info)}
+
+
+def roofline(alg_bytes, ms, traffic):
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+
+def run_matched(L, args, t_start):
+    from fast_ldpc_decoder_for_gpu_fixed_amd.learned import load_table
+    code = L.code_3_6(4000, seed=1)
+    N, E, K = code.n_vars, code.n_edges, code.n_info
+    frames = args.frames or (1 << 20)
+    if frames % 4:
+        raise SystemExit("frames must be a multiple of 4")
+    variant, table = load_table(args.table)
+    sigma = L.channel_sigma(args.ebn0, code.rate)
+    d_llr = L.DeviceBuffer(frames * N)
+    L.channel_generate(d_llr.ptr, N, frames, args.seed, 0, sigma)  # frame-major int8, Philox AWGN
+    L.synchronize()
+
+    # learned: frame-queue decoder, per-frame stop, max `iters`
+    pool = L.PoolDecoder(code, variant, args.iters, table=table, words_per_block=args.words_per_block,
+                         threads=args.threads)
+    d_hard_l, d_it = L.DeviceBuffer(frames * N), L.DeviceBuffer(4 * frames)
+    ms_l = timed(L, pool.stream, args.steps, args.warmup,
+                 lambda: pool.decode_device(d_llr.ptr, frames, d_hard_l.ptr, d_it.ptr))
+    its = d_it.download(np.empty(frames, np.int32))
+    learned = counts(L, L.make_decoder(code, 4, L.OMS), d_hard_l.ptr, frames, K)
+    iters_sum = int(its.sum(dtype=np.int64))
+    alg_l = iters_sum * (2 * E + 2 * N) + frames * 2 * N
+    gbps_l = frames * K / (ms_l * 1e-3) / 1e9
+    learned.update({"info_gbps": round(gbps_l, 3), "ms_per_step": round(ms_l, 4),
+                    "avg_iterations": round(iters_sum / frames, 4), "max_iterations": args.iters,
+                    "decoder": "frame-queue (persistent, per-frame stop, refill)",
+                    "table_deg6": [int(x) for x in table[:, min(6, table.shape[1] - 1)]]})
+    learned.update(pool.config())
+    del d_hard_l
+
+    plain = None
+    if not args.skip_plain:
+        dec = L.make_decoder(code, frames // 4, L.OMS, early_term=0)
+        d_hard_p = L.DeviceBuffer(frames * N)
+        ms_p = timed(L, dec.stream, args.steps, args.warmup,
+                     lambda: dec.decode_device(d_llr.ptr, d_hard_p.ptr, args.iters))
+        plain = counts(L, dec, d_hard_p.ptr, frames, K)
+        alg_p = frames * (args.iters * (2 * E + 2 * N) + 2 * N)
+        plain.update({"info_gbps": round(frames * K / (ms_p * 1e-3) / 1e9, 3), "ms_per_step": round(ms_p, 4),
+                      "avg_iterations": args.iters, "decoder": "drop-in, fixed iterations"})
+        plain["roofline"] = roofline(alg_p, ms_p, traffic_for(args.traffic_json,
+                                                             f"oms_3_6_n4000_f{frames}_it{args.iters}"))
+        cfg = dec.config()
+        plain.update({"words_per_block": cfg["words_per_block"], "threads": cfg["threads"],
+                      "single_degree_kernel": cfg["uniform_deg"] > 0, "batch": cfg["batch"]})
+        del dec, d_hard_p
+
+    key = f"loms_pool_3_6_n4000_f{frames}_ebn0_{args.ebn0:.2f}_max{args.iters}"
+    out = {
+        "metric": "info Gbit/s at matched FER, learned vs plain OMS, 1 MI355X; HBM GB/s",
+        "value": round(gbps_l, 3),
+        "unit": "info Gbit/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_l, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8x4 (u32 words, saturating int8 lanes)",
+        "data": "synthetic: all-zero codeword, BPSK bit0 -> -1, device Philox AWGN at %.2f dB (seed %d), "
+                "LLR sat127(round(16y/sigma^2))" % (args.ebn0, args.seed),
+        "config": {"workload": "learned-offset OMS vs plain 20-iteration OMS, (3,6) N=4000 K=2000, "
+                               "%d frames resident in HBM, Eb/N0 %.2f dB" % (frames, args.ebn0),
+                   "frames": frames, "ebn0_db": args.ebn0, "parallelism": "single GPU",
+                   "learned": learned, "plain": plain},
+        "roofline": roofline(alg_l, ms_l, traffic_for(args.traffic_json, key)),
+    }
+    if plain:
+        out["config"]["speedup_vs_plain"] = round(gbps_l / plain["info_gbps"], 3)
+        out["config"]["fer_matched"] = learned["fer"] <= plain["fer"]
+    if not args.no_cpu_baseline:
+        n = 4 * args.cpu_baseline_words * N
+        llr = d_llr.download(np.empty(n, np.int8))
+        out["cpu_baseline"] = cpu_baseline(code, llr, args.cpu_baseline_words, args.iters)
+    out["wall_s"] = round(time.perf_counter() - t_start, 3)
+    return out
+
+
+def run_plain(L, args, t_start):
+    variant = {"MS": L.MS, "OMS": L.OMS, "OMS_V2": L.OMS_V2, "NMS": L.NMS, "2NMS": L.NMS2}[args.variant.upper()]
+    code = L.code_3_6(4000, seed=1)
+    N, E, K = code.n_vars, code.n_edges, code.n_info
+    frames = args.frames or (1 << 16)
+    W = frames // 4
+    dec = L.make_decoder(code, W, variant, early_term=0, words_per_block=args.words_per_block,
+                         threads=args.threads)
+    cfg = dec.config()
+    sigma = L.channel_sigma(args.ebn0, code.rate)
+    d_in = L.DeviceBuffer(frames * N)
+    # OMS_v2 takes its input in the device word layout (the reference's xMS convention)
+    L.channel_generate(d_in.ptr, N, frames, args.seed, 0, sigma, layout=1 if variant == L.OMS_V2 else 0)
+    d_out = L.DeviceBuffer(frames * N)
+    ms = timed(L, dec.stream, args.steps, args.warmup, lambda: dec.decode_device(d_in.ptr, d_out.ptr, args.iters))
+    info_gbps = frames * K / (ms * 1e-3) / 1e9
+    alg = frames * (args.iters * (2 * E + 2 * N) + 2 * N)
+    out = {
+        "metric": "info Gbit/s (plain %s, (3,6) N=4000, %d it, %d frames, 1 MI355X); HBM GB/s"
+                  % (L.VARIANT_NAMES[variant], args.iters, frames),
+        "value": round(info_gbps, 3),
+        "unit": "info Gbit/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8x4 (u32 words, saturating int8 lanes)",
+        "data": "synthetic: all-zero codeword, BPSK, device Philox AWGN at %.1f dB, LLR sat127(round(16y/sigma^2))"
+                % args.ebn0,
+        "config": {"workload": "drop-in GPU decoder, (3,6) N=4000 K=2000, %s, %d fixed iterations, "
+                               "%d frames resident in HBM" % (L.VARIANT_NAMES[variant], args.iters, frames),
+                   "frames": frames, "iterations": args.iters, "variant": L.VARIANT_NAMES[variant],
+                   "words_per_block": cfg["words_per_block"], "threads": cfg["threads"],
+                   "single_degree_kernel": cfg["uniform_deg"] > 0, "batch": cfg["batch"],
+                   "levels_per_iteration": cfg["n_levels"], "parallelism": "single GPU"},
+        "roofline": roofline(alg, ms, traffic_for(args.traffic_json, f"oms_3_6_n4000_f{frames}_it{args.iters}")
+                             if variant == L.OMS else None),
+    }
+    if not args.no_cpu_baseline and variant == L.OMS:
+        n = 4 * args.cpu_baseline_words * N
+        llr = d_in.download(np.empty(n, np.int8))
+        out["cpu_baseline"] = cpu_baseline(code, llr, args.cpu_baseline_words, args.iters)
+    out["wall_s"] = round(time.perf_counter() - t_start, 3)
+    return out
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames shard "
-              f"across GPUs with no exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames shard across GPUs "
+              f"with no exchange; the multi-GPU layout is a design note only)", file=sys.stderr)
         sys.exit(2)
     import fast_ldpc_decoder_for_gpu_fixed_amd as L
 
@@ -83,79 +261,8 @@ def main():
         L.synchronize()
         print(json.dumps({"copy_bytes": args.copy_calibration}))
         return
-
-    variant = {"MS": L.MS, "OMS": L.OMS, "OMS_V2": L.OMS_V2, "NMS": L.NMS, "2NMS": L.NMS2}[args.variant.upper()]
-    code = L.code_3_6(4000, seed=1)
-    N, E, K = code.n_vars, code.n_edges, code.n_info
-    frames = args.frames
-    W = frames // 4
-    llr = channel_llr(N, frames, args.ebn0)
-    dec = L.make_decoder(code, W, variant, early_term=0, words_per_block=args.words_per_block,
-                         threads=args.threads)
-    cfg = dec.config()
-    d_in = L.DeviceBuffer(llr.nbytes)
-    d_out = L.DeviceBuffer(llr.nbytes)
-    x = llr
-    if variant == L.OMS_V2:
-        from oracle import oracle as O  # layout helper only (input preparation)
-        x = O.interleave(llr, N).view(np.int8)
-    d_in.upload(x)
-    stream = dec.stream
-
-    for _ in range(args.warmup):
-        dec.decode_device(d_in.ptr, d_out.ptr, args.iters)
-    L.synchronize()
-    timer = L.CTimer(stream=stream)
-    L.synchronize()
     t0 = time.perf_counter()
-    timer.start()
-    for _ in range(args.steps):
-        dec.decode_device(d_in.ptr, d_out.ptr, args.iters)
-    timer.stop()
-    ms_total = timer.get_time_ms()
-    L.synchronize()
-    wall = time.perf_counter() - t0
-
-    ms_step = ms_total / args.steps
-    info_gbps = frames * K / (ms_step * 1e-3) / 1e9
-    alg_bytes = frames * (args.iters * (2 * E + 2 * N) + 2 * N)
-    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
-
-    traffic = None
-    tp = Path(args.traffic_json)
-    if tp.exists():
-        try:
-            t = json.loads(tp.read_text())
-            if t.get("workload") == f"oms_3_6_n4000_f{frames}_it{args.iters}":
-                traffic = t.get("bytes_per_launch")
-        except (ValueError, OSError):
-            traffic = None
-
-    out = {
-        "metric": "info Gbit/s (plain OMS, (3,6) N=4000, 20 it, 2^16 frames, 1 MI355X); HBM GB/s",
-        "value": round(info_gbps, 3),
-        "unit": "info Gbit/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int8x4 (u32 words, saturating int8 lanes)",
-        "data": "synthetic: all-zero codeword, BPSK, AWGN at %.1f dB, LLR sat127(round(16y/sigma^2))" % args.ebn0,
-        "config": {"workload": "drop-in GPU decoder, (3,6) N=4000 K=2000, %s, %d fixed iterations, "
-                               "%d frames resident in HBM" % (L.VARIANT_NAMES[variant], args.iters, frames),
-                   "frames": frames, "iterations": args.iters, "variant": L.VARIANT_NAMES[variant],
-                   "words_per_block": cfg["words_per_block"], "threads": cfg["threads"],
-                   "single_degree_kernel": cfg["uniform_deg"] > 0, "batch": cfg["batch"],
-                   "levels_per_iteration": cfg["n_levels"], "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "wall_s": round(wall, 3),
-    }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(code, llr, args.cpu_baseline_words, args.iters, variant)
+    out = run_matched(L, args, t0) if args.mode == "matched" else run_plain(L, args, t0)
     print(json.dumps(out), flush=True)
 
 

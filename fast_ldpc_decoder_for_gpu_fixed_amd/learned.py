@@ -76,6 +76,13 @@ def evaluate(code: Code, table: np.ndarray, variant: int, iters: int, ebn0_list,
     return out
 
 
+def load_table(path: str):
+    """(variant, int8 table) from a file written with --save-table."""
+    with open(path) as f:
+        rec = json.load(f)
+    return (LOMS if rec["variant"] == "LOMS" else LNMS), np.asarray(rec["table"], dtype=np.int8)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--mode", choices=["offset", "scale"], default="offset")
@@ -89,6 +96,8 @@ def main(argv=None):
     ap.add_argument("--eval-frames", type=int, default=1 << 22)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--eval-seed", type=int, default=777)
+    ap.add_argument("--save-table", default="", help="write the quantised table and its training record (JSON)")
+    ap.add_argument("--no-eval", action="store_true")
     a = ap.parse_args(argv)
 
     code = code_3_6(4000, seed=1)
@@ -103,11 +112,19 @@ def main(argv=None):
         "code": "(3,6) N=4000 K=2000, colour-ordered table, seed 1",
         "training": {"mode": a.mode, "iterations": a.iters, "batch_frames": a.batch, "steps": a.steps,
                      "frames": a.batch * a.steps, "lr": a.lr, "ebn0_range_db": [a.ebn0_lo, a.ebn0_hi],
-                     "seconds": round(secs, 2), "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+                     "seed": a.seed, "seconds": round(secs, 2), "loss_first": float(hist[0]),
+                     "loss_last": float(hist[-1]),
                      "float_table_deg%d" % col: [round(float(x), 4) for x in params[:, col]],
                      "int8_table_deg%d" % col: [int(x) for x in table[:, col]]},
-        "evaluation": evaluate(code, table, variant, a.iters, a.eval_ebn0, a.eval_frames, a.eval_seed),
     }
+    if a.save_table:
+        rec = {"variant": "LOMS" if variant == LOMS else "LNMS", "code": res["code"],
+               "table": table.astype(int).tolist(), "training": res["training"]}
+        with open(a.save_table, "w") as f:
+            json.dump(rec, f, indent=1)
+            f.write("\n")
+    if not a.no_eval:
+        res["evaluation"] = evaluate(code, table, variant, a.iters, a.eval_ebn0, a.eval_frames, a.eval_seed)
     json.dump(res, sys.stdout, indent=1)
     print()
 
