@@ -410,6 +410,73 @@ class PoolDecoder:
                 "batch": B.value}
 
 
+def pool_autotune(code: Code, variant: int, max_iterations: int, ebn0_db: float = 2.5,
+                  sample_frames: int = 32768) -> dict:
+    """Fastest frame-queue tuning for this code (words per workgroup, workgroup
+    size, single-degree path and its items in flight), timed on the GPU."""
+    tu, ms, n = Tuning(), C.c_double(), C.c_int()
+    check(load().ldpc_pool_autotune(code.handle, variant, max_iterations, ebn0_db, sample_frames, C.byref(tu),
+                                    C.byref(ms), C.byref(n)), "pool_autotune")
+    return {"words_per_block": tu.words_per_block, "threads": tu.threads, "uniform": tu.uniform,
+            "batch": tu.batch, "ms": ms.value, "tried": n.value}
+
+
+class MixedDecoder:
+    """One decode step over frames of several codes (one frame-queue decoder
+    per code, side by side on the GPU, optionally replayed from a hipGraph)."""
+
+    def __init__(self, codes, variant: int, max_iterations: int, tunings=None, max_frames: int = 0):
+        self.codes = list(codes)
+        hs = (C.c_void_p * len(self.codes))(*[c.handle.value for c in self.codes])
+        tarr = None
+        if tunings is not None:
+            tarr = (Tuning * len(self.codes))()
+            for i, t in enumerate(tunings):
+                t = t or {}
+                tarr[i] = Tuning(t.get("words_per_block", 0), t.get("threads", 0), -1, -1, t.get("uniform", -1),
+                                 t.get("batch", 0))
+        h = load().ldpc_mixed_create(hs, len(self.codes), variant, max_iterations, tarr, max_frames)
+        if not h:
+            raise LdpcError(f"ldpc_mixed_create failed: {_lib.last_error()}")
+        self._h = C.c_void_p(h)
+        self.variant, self.max_iterations = variant, max_iterations
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.ldpc_mixed_destroy(h)
+            self._h = None
+
+    def set_table(self, code_index: int, table) -> None:
+        t = np.ascontiguousarray(table, dtype=np.int8)
+        assert t.ndim == 2
+        check(load().ldpc_mixed_set_table(self._h, code_index, ptr(t, C.c_int8), t.shape[0], t.shape[1]),
+              "mixed_set_table")
+
+    def decode_device(self, d_in: int, d_offsets: int, d_code_ids: int, frames: int, d_out: int, d_iters: int = 0,
+                      graph: bool = False, stream: int = 0) -> None:
+        v = lambda p: C.c_void_p(p) if p else None
+        check(load().ldpc_mixed_decode(self._h, v(d_in), v(d_offsets), v(d_code_ids), frames, v(d_out), v(d_iters),
+                                       int(graph), v(stream)), "mixed_decode")
+
+    def config(self, code_index: int) -> dict:
+        P, T, G, U, B = (C.c_int() for _ in range(5))
+        check(load().ldpc_mixed_config(self._h, code_index, C.byref(P), C.byref(T), C.byref(G), C.byref(U),
+                                       C.byref(B)), "mixed_config")
+        return {"words_per_block": P.value, "threads": T.value, "max_workgroups": G.value, "uniform_deg": U.value,
+                "batch": B.value}
+
+
+def mixed_layout(codes, code_ids):
+    """Byte offsets of frames laid out back to back in code_ids order:
+    (offsets uint64 [F], total bytes)."""
+    n = np.array([c.n_vars for c in codes], np.uint64)[np.asarray(code_ids, np.int64)]
+    off = np.zeros(len(n), np.uint64)
+    if len(n) > 1:
+        np.cumsum(n[:-1], out=off[1:])
+    return off, int(n.sum())
+
+
 def channel_sigma(snr_db: float, rate: float, es_n0: bool = False) -> float:
     return float(load().ldpc_channel_sigma(snr_db, rate, int(es_n0)))
 

@@ -82,6 +82,15 @@ SIGNATURES = {
     "ldpc_pool_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ldpc_pool_simulate": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,
                                      C.POINTER(SimStats)]),
+    "ldpc_pool_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(Tuning),
+                                     C.POINTER(C.c_double), _c_int_p]),
+    "ldpc_mixed_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
+                                       C.c_uint64]),
+    "ldpc_mixed_set_table": (C.c_int, [C.c_void_p, C.c_int, _i8_p, C.c_int, C.c_int]),
+    "ldpc_mixed_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                    C.c_void_p, C.c_int, C.c_void_p]),
+    "ldpc_mixed_config": (C.c_int, [C.c_void_p, C.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
+    "ldpc_mixed_destroy": (None, [C.c_void_p]),
     "ldpc_pool_config": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "ldpc_pool_stream": (C.c_void_p, [C.c_void_p]),
     "ldpc_pool_destroy": (None, [C.c_void_p]),

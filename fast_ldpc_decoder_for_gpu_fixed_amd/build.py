@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 SOURCES = ["decode_kernels.hip", "sim_kernels.hip", "learn_kernels.hip", "pool_kernels.hip", "api.hip",
-           "learn_api.hip", "pool_api.hip", "code.cpp"]
+           "learn_api.hip", "pool_api.hip", "mixed_api.hip", "code.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

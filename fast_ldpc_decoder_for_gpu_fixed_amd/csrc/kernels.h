@@ -59,11 +59,14 @@ struct PoolArgs {
     const int *level_beg;
     int n_levels, N, C, E;
     int P, log2P;
-    const int8_t *in_frames;      // [total][N] frame-major LLRs
-    uint8_t *out_hard;            // [total][N] decisions
-    int *out_iters;               // [total] iterations per frame, or null
+    const int8_t *in_frames;      // LLRs: frame f at in_frames + off(f) (N bytes)
+    uint8_t *out_hard;            // decisions: frame f at out_hard + off(f)
+    int *out_iters;               // [frames] iterations per frame, or null
     unsigned long long *next;     // queue head (device counter, starts at 0)
-    unsigned long long total;     // frames in the queue
+    unsigned long long total;     // frames in the queue (when total_ptr is null)
+    const unsigned *total_ptr;    // device count of queued frames, or null
+    const unsigned *frame_list;   // queue position -> frame id, or null (identity)
+    const unsigned long long *frame_off;  // frame id -> byte offset, or null (f * N)
     uint32_t *msgs;               // [grid][E][P] per-workgroup message words
     int max_iters, variant;
     const int8_t *ltable;         // learned table or null
@@ -75,7 +78,10 @@ struct PoolArgs {
 // slot bookkeeping and the learned table take the rest of the 160 KiB
 constexpr size_t POOL_LDS_DYNAMIC = MAX_LDS_BYTES - 8192;
 hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s);
-int pool_blocks_per_cu(const PoolArgs &a, int dmax, int threads);
+// sets the LDS limit of the instantiation `a` selects (call before any launch
+// and outside graph capture) and returns how many of its workgroups a CU holds
+int pool_prepare(const PoolArgs &a, int dmax, int threads);
+hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t s);
 
 // float shadow decoder (learned tables)
 struct ShadowArgs {

@@ -1,0 +1,40 @@
+// pool.h -- the frame-queue decoder object, shared by its C ABI and the
+// mixed-code decoder built from one such object per code.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "code.h"
+#include "kernels.h"
+
+struct ldpc_pool {
+    ldpc_code code;
+    ldpc::Plan plan;
+    int variant = 0, max_iters = 0, P = 1, log2P = 0, threads = 256, max_grid = 1;
+    hipStream_t stream = nullptr;
+    uint2 *d_checks = nullptr;
+    uint32_t *d_edge_var = nullptr;
+    int *d_level_beg = nullptr;
+    uint32_t *d_msgs = nullptr;
+    unsigned long long *d_next = nullptr;
+    int8_t *d_table = nullptr;
+    int tits = 0, tdeg = 0;
+    uint32_t *d_edge_off16 = nullptr;  // single-degree path
+    int uniform_deg = 0, batch = 1;
+};
+
+
+namespace ldpc {
+// where one launch takes its frames from and writes them to
+struct PoolIO {
+    const int8_t *in = nullptr;
+    uint8_t *out = nullptr;
+    int *iters = nullptr;
+    unsigned long long total = 0;
+    const unsigned *total_ptr = nullptr;
+    const unsigned *frame_list = nullptr;
+    const unsigned long long *frame_off = nullptr;
+};
+PoolArgs pool_args(const ldpc_pool *p);
+// zero the pool's queue head and launch its kernel on s (capturable: kernels only)
+hipError_t pool_enqueue(ldpc_pool *p, const PoolIO &io, int grid, hipStream_t s);
+}  // namespace ldpc

@@ -9,26 +9,11 @@
 #include "../../include/ldpc_mi355x.h"
 #include "code.h"
 #include "kernels.h"
+#include "pool.h"
 
 namespace ldpc {
 void set_last_error(const std::string &m);
 }
-
-struct ldpc_pool {
-    ldpc_code code;
-    ldpc::Plan plan;
-    int variant = 0, max_iters = 0, P = 1, log2P = 0, threads = 256, max_grid = 1;
-    hipStream_t stream = nullptr;
-    uint2 *d_checks = nullptr;
-    uint32_t *d_edge_var = nullptr;
-    int *d_level_beg = nullptr;
-    uint32_t *d_msgs = nullptr;
-    unsigned long long *d_next = nullptr;
-    int8_t *d_table = nullptr;
-    int tits = 0, tdeg = 0;
-    uint32_t *d_edge_off16 = nullptr;  // single-degree path
-    int uniform_deg = 0, batch = 1;
-};
 
 #define PHIP(expr)                                                                          \
     do {                                                                                    \
@@ -54,8 +39,10 @@ void pool_free(ldpc_pool *p) {
 }
 
 __global__ void k_zero_counter(unsigned long long *p) { *p = 0ull; }
+}  // namespace
 
-ldpc::PoolArgs pool_args(const ldpc_pool *p) {
+namespace ldpc {
+PoolArgs pool_args(const ldpc_pool *p) {
     ldpc::PoolArgs a;
     std::memset(&a, 0, sizeof a);
     a.checks = p->d_checks;
@@ -79,7 +66,22 @@ ldpc::PoolArgs pool_args(const ldpc_pool *p) {
     a.batch = p->batch;
     return a;
 }
-}  // namespace
+
+hipError_t pool_enqueue(ldpc_pool *p, const PoolIO &io, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, p->d_next);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    PoolArgs a = pool_args(p);
+    a.in_frames = io.in;
+    a.out_hard = io.out;
+    a.out_iters = io.iters;
+    a.total = io.total;
+    a.total_ptr = io.total_ptr;
+    a.frame_list = io.frame_list;
+    a.frame_off = io.frame_off;
+    return launch_pool(a, p->code.max_deg, grid, p->threads, s);
+}
+}  // namespace ldpc
 
 extern "C" {
 
@@ -159,8 +161,14 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     int dev = 0;
     hipGetDevice(&dev);
     hipGetDeviceProperties(&prop, dev);
-    ldpc::PoolArgs a = pool_args(p);
-    p->max_grid = prop.multiProcessorCount * ldpc::pool_blocks_per_cu(a, code->max_deg, T);
+    const ldpc::PoolArgs a = ldpc::pool_args(p);
+    const int per_cu = ldpc::pool_prepare(a, code->max_deg, T);
+    if (per_cu < 1) {
+        ldpc::set_last_error("the frame-queue kernel cannot run with this words_per_block / threads");
+        pool_free(p);
+        return nullptr;
+    }
+    p->max_grid = prop.multiProcessorCount * per_cu;
     std::vector<uint2> ck((size_t)code->C);
     for (int i = 0; i < code->C; i++) ck[i] = make_uint2(p->plan.check_info[i], p->plan.check_e0[i]);
     hipError_t e = hipSuccess;
@@ -224,14 +232,12 @@ int ldpc_pool_decode(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t 
     const uint64_t slots = 4ull * (uint64_t)p->P;
     int grid = workgroups > 0 ? std::min(workgroups, p->max_grid) : p->max_grid;
     grid = (int)std::min<uint64_t>((uint64_t)grid, (frames + slots - 1) / slots);
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, p->d_next);
-    PHIP(hipGetLastError());
-    ldpc::PoolArgs a = pool_args(p);
-    a.in_frames = d_in;
-    a.out_hard = d_out;
-    a.out_iters = d_iters;
-    a.total = frames;
-    PHIP(ldpc::launch_pool(a, p->code.max_deg, grid, p->threads, s));
+    ldpc::PoolIO io;
+    io.in = d_in;
+    io.out = d_out;
+    io.iters = d_iters;
+    io.total = frames;
+    PHIP(ldpc::pool_enqueue(p, io, grid, s));
     return LDPC_OK;
 }
 
@@ -308,6 +314,91 @@ int ldpc_pool_simulate(ldpc_pool *p, double ebn0_db, int quant, uint64_t seed, u
     out->syndrome_failures = cnt[2];
     out->avg_iterations = it_sum / (double)n_frames;
     out->decode_ms = ms;
+    return LDPC_OK;
+}
+
+int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, double ebn0_db, uint64_t sample_frames,
+                       ldpc_tuning *best, double *best_ms, int *n_tried) {
+    if (!code || !best || max_iterations < 1 || sample_frames == 0) {
+        ldpc::set_last_error("ldpc_pool_autotune: bad arguments");
+        return LDPC_ERR_ARG;
+    }
+    const double rate = (double)(code->N - code->C) / (double)code->N;
+    const double sigma = ldpc_channel_sigma(ebn0_db, rate, 0);
+    int8_t *d_in = nullptr;
+    uint8_t *d_out = nullptr;
+    int *d_it = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc((void **)&d_in, (size_t)sample_frames * code->N);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_out, (size_t)sample_frames * code->N);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_it, (size_t)sample_frames * sizeof(int));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = ldpc::launch_channel(d_in, 0, code->N, sample_frames, 0, 0x7475E5EEDull, 0, sigma, 0, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    std::vector<int8_t> table((size_t)max_iterations * (ldpc::MAX_DEG + 1), variant == LDPC_LNMS ? 6 : 1);
+    double best_t = 1e300;
+    int tried = 0;
+    std::string last = "no configuration could run";
+    for (int P = 1; e == hipSuccess && P <= ldpc::MAX_P; P *= 2) {
+        if ((size_t)code->N * 4 * P > ldpc::POOL_LDS_DYNAMIC) break;
+        for (int T = 64; e == hipSuccess && T <= 1024; T *= 2) {
+            if (T % P || T < 4 * P) continue;
+            for (int uni = 1; uni >= 0; uni--) {
+                for (int batch : {1, 2, 4}) {
+                    if (!uni && batch != 1) continue;
+                    ldpc_tuning tu = {P, T, -1, -1, uni, batch};
+                    ldpc_pool *pool = ldpc_pool_create(code, variant, max_iterations, &tu);
+                    if (!pool) continue;  // this combination does not qualify
+                    if (uni && !pool->uniform_deg) {
+                        ldpc_pool_destroy(pool);
+                        continue;
+                    }
+                    int rc = LDPC_OK;
+                    if (variant == LDPC_LOMS || variant == LDPC_LNMS)
+                        rc = ldpc_pool_set_table(pool, table.data(), max_iterations, ldpc::MAX_DEG + 1);
+                    float t_min = 1e30f;
+                    for (int rep = 0; rc == LDPC_OK && rep < 3; rep++) {
+                        hipEventRecord(e0, pool->stream);
+                        rc = ldpc_pool_decode(pool, d_in, sample_frames, d_out, d_it, 0, nullptr);
+                        hipEventRecord(e1, pool->stream);
+                        if (rc == LDPC_OK && hipEventSynchronize(e1) == hipSuccess) {
+                            float t = 0.f;
+                            hipEventElapsedTime(&t, e0, e1);
+                            if (rep > 0) t_min = std::min(t_min, t);  // the first run warms up
+                        }
+                    }
+                    ldpc_pool_destroy(pool);
+                    if (rc != LDPC_OK) {
+                        last = ldpc_last_error();
+                        continue;
+                    }
+                    tried++;
+                    if (t_min < best_t) {
+                        best_t = t_min;
+                        *best = tu;
+                        best->batch = uni ? batch : 0;
+                    }
+                }
+                if (!uni) break;
+            }
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(d_in);
+    hipFree(d_out);
+    hipFree(d_it);
+    if (e != hipSuccess) {
+        ldpc::set_last_error(std::string("ldpc_pool_autotune: ") + hipGetErrorString(e));
+        return LDPC_ERR_HIP;
+    }
+    if (!tried) {
+        ldpc::set_last_error("ldpc_pool_autotune: " + last);
+        return LDPC_ERR_ARG;
+    }
+    if (best_ms) *best_ms = best_t;
+    if (n_tried) *n_tried = tried;
     return LDPC_OK;
 }
 

@@ -65,6 +65,7 @@ template <bool EQ, int DMAXC, int UD, int G>
 __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_pool(const PoolArgs a) {
     extern __shared__ uint32_t lds_post[];
     __shared__ long long s_fid[POOL_SLOTS];
+    __shared__ unsigned long long s_off[POOL_SLOTS];
     __shared__ int s_it[POOL_SLOTS];
     __shared__ int s_flag[POOL_SLOTS];  // 1: stop this iteration, 2: refilled
     __shared__ uint32_t s_keep[MAX_P];
@@ -95,12 +96,13 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
         if (tid == 0) {
             int n = 0;
             for (int s = 0; s < S; s++) n += s_flag[s] == 1;
+            const unsigned long long total = a.total_ptr ? (unsigned long long)*a.total_ptr : a.total;
             unsigned long long base = n ? atomicAdd(a.next, (unsigned long long)n) : 0ull;
             int active = 0;
             for (int s = 0; s < S; s++) {
                 if (s_flag[s] == 1) {
-                    if (base < a.total) {
-                        s_fid[s] = (long long)base;
+                    if (base < total) {
+                        s_fid[s] = (long long)base;  // queue position; the frame id follows below
                         s_it[s] = 0;
                         s_flag[s] = 2;
                         s_keep[s >> 2] &= ~(0xFFu << (8 * (s & 3)));
@@ -118,10 +120,18 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
         }
         __syncthreads();
         if (s_active == 0) break;
+        // queue position -> frame id and its byte offset, one slot per thread
+        if (tid < S && s_flag[tid] == 2) {
+            const unsigned long long q = (unsigned long long)s_fid[tid];
+            const unsigned long long f = a.frame_list ? (unsigned long long)a.frame_list[q] : q;
+            s_fid[tid] = (long long)f;
+            s_off[tid] = a.frame_off ? a.frame_off[f] : f * (unsigned long long)N;
+        }
+        __syncthreads();
         // ---- load the LLRs of the refilled frames into their lanes
         for (int s = 0; s < S; s++) {
             if (s_flag[s] != 2) continue;  // uniform: flags are read after the barrier
-            const int8_t *src = a.in_frames + (size_t)s_fid[s] * N;
+            const int8_t *src = a.in_frames + s_off[s];
             const int wl = s >> 2, k = s & 3;
             for (int v = tid; v < N; v += T) pb[((size_t)v * P + wl) * 4 + k] = (uint8_t)src[v];
         }
@@ -200,8 +210,8 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
             if (s_flag[s] != 1 || s_fid[s] < 0) continue;
             const size_t f = (size_t)s_fid[s];
             const int wl = s >> 2, k = s & 3;
-            uint8_t *dst = a.out_hard + f * N;
-            if ((N & 3) == 0) {
+            uint8_t *dst = a.out_hard + s_off[s];
+            if ((N & 3) == 0 && (s_off[s] & 3) == 0) {
                 uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
                 for (int q = tid; q < (N >> 2); q += T) {
                     uint32_t x = 0;
@@ -221,13 +231,6 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
 
 template <bool EQ, int DMAXC, int UD, int G>
 static hipError_t launch_pool_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pool<EQ, DMAXC, UD, G>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)POOL_LDS_DYNAMIC);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
     hipLaunchKernelGGL((k_pool<EQ, DMAXC, UD, G>), dim3(grid), dim3(threads), (size_t)a.N * a.P * 4, s, a);
     return hipGetLastError();
 }
@@ -267,12 +270,23 @@ hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipSt
                                  : launch_pool_eq<false>(a, dmax, ud, a.batch, grid, threads, s);
 }
 
-int pool_blocks_per_cu(const PoolArgs &a, int dmax, int threads) {
+int pool_prepare(const PoolArgs &a, int dmax, int threads) {
     int n = 0;
     const int ud = a.uniform_deg;
     const void *f = a.variant == VAR_2NMS ? pool_kernel<true>(dmax, ud, a.batch) : pool_kernel<false>(dmax, ud, a.batch);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, (size_t)a.N * a.P * 4) != hipSuccess) return 1;
-    return n > 0 ? n : 1;
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)POOL_LDS_DYNAMIC) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, (size_t)a.N * a.P * 4) != hipSuccess) return 0;
+    return n;
+}
+
+__global__ void k_zero_u32(unsigned *p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, p, n);
+    return hipGetLastError();
 }
 
 }  // namespace ldpc

@@ -250,12 +250,42 @@ int ldpc_pool_decode(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8
  * avg_iterations is per frame; decode_ms covers the decode launches only. */
 int ldpc_pool_simulate(ldpc_pool *pool, double ebn0_db, int quant, uint64_t seed, uint32_t snr_idx,
                        uint64_t n_frames, uint64_t batch_frames, ldpc_sim_stats *out);
+/* Autotuner (no counterpart in the reference): times the frame-queue
+ * decoder on sample_frames channel frames at ebn0_db for every words-per-
+ * workgroup P (posterior block N*P*4 bytes of LDS), workgroup size and, on the
+ * single-degree path, items in flight per thread (1, 2, 4); writes the fastest
+ * into *best (words_per_block, threads, uniform, batch set; other fields -1).
+ * Learned variants are timed with a constant table. */
+int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, double ebn0_db,
+                       uint64_t sample_frames, ldpc_tuning *best, double *best_ms, int *n_tried);
 /* Words per workgroup, workgroup size, resident workgroups, the degree of
  * the single-degree path (0 = generic path) and its items per thread. */
 int ldpc_pool_config(const ldpc_pool *pool, int *words_per_block, int *threads, int *max_workgroups,
                      int *uniform_deg, int *batch);
 void *ldpc_pool_stream(ldpc_pool *pool);
 void ldpc_pool_destroy(ldpc_pool *pool);
+
+/* ---- mixed-code decode step (no single counterpart in the reference, whose
+ * code is fixed per binary: src/matrix/code.h:1) ------------------------------
+ * One frame-queue decoder per code (tunings[c], e.g. from ldpc_pool_autotune,
+ * or NULL = defaults), all with the same variant and iteration limit.  A step
+ * decodes `frames` frames held in one byte buffer: frame f belongs to code
+ * d_code_ids[f] and sits at d_in + d_offsets[f] (its N bytes of LLRs); its
+ * decisions go to d_out + d_offsets[f] and its iteration count to d_iters[f]
+ * (0 for a frame whose code id is not a code of this object).  The step is
+ * asynchronous on `stream` (NULL = the null stream); use_graph = 1 captures it
+ * into a hipGraph on first use and replays the graph while the six buffer
+ * arguments stay the same (a change, or a new table, recaptures).
+ * max_frames sizes the per-code queues (0 = on first use). */
+typedef struct ldpc_mixed ldpc_mixed;
+ldpc_mixed *ldpc_mixed_create(const ldpc_code *const *codes, int n_codes, int variant, int max_iterations,
+                              const ldpc_tuning *tunings, uint64_t max_frames);
+int ldpc_mixed_set_table(ldpc_mixed *mixed, int code_index, const int8_t *table, int n_iters, int n_degs);
+int ldpc_mixed_decode(ldpc_mixed *mixed, const int8_t *d_in, const uint64_t *d_offsets, const uint8_t *d_code_ids,
+                      uint64_t frames, uint8_t *d_out, int *d_iters, int use_graph, void *stream);
+int ldpc_mixed_config(const ldpc_mixed *mixed, int code_index, int *words_per_block, int *threads,
+                      int *max_workgroups, int *uniform_deg, int *batch);
+void ldpc_mixed_destroy(ldpc_mixed *mixed);
 
 /* ---- learned tables: float shadow decoder trained on the GPU ----------------
  * (no counterpart in the reference).  The shadow runs the integer decoder's
