@@ -13,6 +13,12 @@ FERs come from the device error counters on the same frames.
 --mode plain: the drop-in decoder alone on 2^16 frames, 20 fixed iterations
 (the plain-OMS HBM roofline line).
 
+--mode mixed: the mixed-code step on two quasi-cyclic codes (12x24 base graph,
+Z=81, N=1944, rate 1/2; 4x24, Z=96, N=2304, rate 5/6), 2^17 frames interleaved
+50/50, per-frame Eb/N0 uniform in [1.5, 3.5] dB, OMS (clamp 31, beta 1), at
+most 30 iterations with per-frame stop; each code's kernel autotuned, the step
+replayed from a hipGraph.
+
 Prints one JSON line: info Gbit/s, the dominant kernel's roofline (algorithmic
 bytes / kernel time from HIP events on the kernel's stream) and the CPU
 baseline (the oracle's restatement of the reference path on the host cores).
@@ -40,7 +46,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", choices=["matched", "plain"], default="matched")
+    ap.add_argument("--mode", choices=["matched", "plain", "mixed"], default="matched")
     ap.add_argument("--variant", default="OMS", help="plain mode: MS, OMS, OMS_V2, NMS, 2NMS")
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--frames", type=int, default=0, help="default: 2^20 (matched), 2^16 (plain)")
@@ -247,6 +253,85 @@ def run_plain(L, args, t_start):
     return out
 
 
+def run_mixed(L, args, t_start):
+    codes = [L.Code.qc(12, 24, 81, 0.35, seed=1), L.Code.qc(4, 24, 96, 0.35, seed=2)]
+    names = ["qc_12x24_z81", "qc_4x24_z96"]
+    frames = args.frames or (1 << 17)
+    max_it = 30
+    variant = L.OMS_V2  # OMS with the 31 clamp on every check (these codes have many check degrees)
+    # frame f belongs to code f & 1; each code's frames are contiguous in its own region
+    cid = (np.arange(frames) & 1).astype(np.uint8)
+    n_c = [(frames + 1 - k) // 2 for k in range(2)]
+    base = [0, n_c[0] * codes[0].n_vars]
+    off = np.empty(frames, np.uint64)
+    for k in range(2):
+        off[k::2] = base[k] + np.arange(n_c[k], dtype=np.uint64) * codes[k].n_vars
+    total = base[1] + n_c[1] * codes[1].n_vars
+    d_in, d_out = L.DeviceBuffer(total), L.DeviceBuffer(total)
+    d_off, d_cid, d_it = L.DeviceBuffer(off.nbytes), L.DeviceBuffer(frames), L.DeviceBuffer(4 * frames)
+    d_off.upload(off)
+    d_cid.upload(cid)
+    for k, c in enumerate(codes):
+        L.channel_generate_range(d_in.ptr + base[k], c.n_vars, n_c[k], args.seed + k, 1.5, 3.5, c.rate)
+    L.synchronize()
+    t_tune = time.perf_counter()
+    tuned = [L.pool_autotune(c, variant, max_it, ebn0_db=2.5, sample_frames=32768) for c in codes]
+    t_tune = time.perf_counter() - t_tune
+    mixed = L.MixedDecoder(codes, variant, max_it, tunings=tuned, max_frames=frames)
+    step = lambda graph: mixed.decode_device(d_in.ptr, d_off.ptr, d_cid.ptr, frames, d_out.ptr, d_it.ptr,
+                                             graph=graph)
+    ms_eager = timed(L, 0, args.steps, args.warmup, lambda: step(False))
+    ms = timed(L, 0, args.steps, args.warmup, lambda: step(True))
+    its = d_it.download(np.empty(frames, np.int32))
+    per_code, alg, info = [], 0, 0
+    for k, c in enumerate(codes):
+        ik = its[k::2].astype(np.int64)
+        r = counts(L, L.make_decoder(c, 4, L.OMS_V2), d_out.ptr + base[k], n_c[k], c.n_info)
+        r.update({"code": names[k], "n": c.n_vars, "k": c.n_info, "edges": c.n_edges, "max_check_degree": c.max_deg,
+                  "levels": c.n_levels, "frames": n_c[k], "avg_iterations": round(float(ik.mean()), 4),
+                  "iterations_hist": np.bincount(ik, minlength=max_it + 1).tolist(),
+                  "tuning": {kk: tuned[k][kk] for kk in ("words_per_block", "threads", "uniform", "batch")},
+                  "tune_ms_per_32768_frames": round(tuned[k]["ms"], 4)})
+        per_code.append(r)
+        alg += int(ik.sum()) * (2 * c.n_edges + 2 * c.n_vars) + n_c[k] * 2 * c.n_vars
+        info += n_c[k] * c.n_info
+    gbps = info / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "info Gbit/s, mixed-code frame-queue decode step (2 QC codes), 1 MI355X; HBM GB/s",
+        "value": round(gbps, 3), "unit": "info Gbit/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8x4 (u32 words, saturating int8 lanes)",
+        "data": "synthetic: all-zero codewords, BPSK, device Philox AWGN, per-frame Eb/N0 U[1.5, 3.5] dB",
+        "config": {"workload": "mixed QC 12x24 Z=81 + 4x24 Z=96, %d frames interleaved 50/50, OMS clamp 31, "
+                               "max %d iterations, per-frame stop, hipGraph step" % (frames, max_it),
+                   "frames": frames, "codes": per_code, "eager_ms_per_step": round(ms_eager, 4),
+                   "eager_info_gbps": round(info / (ms_eager * 1e-3) / 1e9, 3),
+                   "autotune_seconds": round(t_tune, 2), "parallelism": "single GPU"},
+        "roofline": roofline(alg, ms, None),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        t_cpu, bits = 0.0, 0
+        for k, c in enumerate(codes):
+            nf = 2048
+            llr = d_in.download(np.empty(nf * c.n_vars, np.int8)) if k == 0 else None
+            if k == 1:
+                raw = np.empty(base[1] + nf * c.n_vars, np.int8)
+                llr = d_in.download(raw)[base[1]:]
+            deg, pos = c.export()
+            t0 = time.perf_counter()
+            O.decode_frames(deg, pos, c.n_vars, O.OMS_V2, llr.reshape(nf, c.n_vars), max_it, threads=threads)
+            t_cpu += time.perf_counter() - t0
+            bits += nf * c.n_info
+        out["cpu_baseline"] = {"value": round(bits / t_cpu / 1e9, 4), "unit": "info Gbit/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"2048 frames of each code from the same input, per-frame stop, max {max_it}, "
+                                         f"{t_cpu:.2f} s on {threads} host threads"}
+    out["wall_s"] = round(time.perf_counter() - t_start, 3)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -262,7 +347,8 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_calibration}))
         return
     t0 = time.perf_counter()
-    out = run_matched(L, args, t0) if args.mode == "matched" else run_plain(L, args, t0)
+    run = {"matched": run_matched, "plain": run_plain, "mixed": run_mixed}[args.mode]
+    out = run(L, args, t0)
     print(json.dumps(out), flush=True)
 
 

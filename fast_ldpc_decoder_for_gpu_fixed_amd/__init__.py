@@ -22,6 +22,7 @@ from . import _lib
 from ._lib import LdpcError, LearnConfig, SimStats, Tuning, check, load, ptr
 
 MS, OMS, OMS_V2, NMS, NMS2, LOMS, LNMS = range(7)
+STOP_SYNDROME, STOP_V2C_PARITY = 0, 1  # frame-queue decoder stop tests
 VARIANT_NAMES = {MS: "MS", OMS: "OMS", OMS_V2: "OMS_v2", NMS: "NMS", NMS2: "2NMS",
                  LOMS: "learned-OMS", LNMS: "learned-NMS"}
 
@@ -345,13 +346,13 @@ class PoolDecoder:
     device queue.  Inputs and outputs stay on the device (frame-major)."""
 
     def __init__(self, code: Code, variant: int, max_iterations: int, words_per_block: int = 0,
-                 threads: int = 0, table=None, uniform: int = -1, batch: int = 0):
+                 threads: int = 0, table=None, uniform: int = -1, batch: int = 0, stop_rule: int = 0):
         tu = Tuning(words_per_block, threads, -1, -1, uniform, batch)
-        h = load().ldpc_pool_create(code.handle, variant, max_iterations, C.byref(tu))
+        h = load().ldpc_pool_create(code.handle, variant, max_iterations, stop_rule, C.byref(tu))
         if not h:
             raise LdpcError(f"ldpc_pool_create failed: {_lib.last_error()}")
         self._h = C.c_void_p(h)
-        self.code, self.variant, self.max_iterations = code, variant, max_iterations
+        self.code, self.variant, self.max_iterations, self.stop_rule = code, variant, max_iterations, stop_rule
         if table is not None:
             self.set_table(table)
 
@@ -411,12 +412,12 @@ class PoolDecoder:
 
 
 def pool_autotune(code: Code, variant: int, max_iterations: int, ebn0_db: float = 2.5,
-                  sample_frames: int = 32768) -> dict:
+                  sample_frames: int = 32768, stop_rule: int = 0) -> dict:
     """Fastest frame-queue tuning for this code (words per workgroup, workgroup
     size, single-degree path and its items in flight), timed on the GPU."""
     tu, ms, n = Tuning(), C.c_double(), C.c_int()
-    check(load().ldpc_pool_autotune(code.handle, variant, max_iterations, ebn0_db, sample_frames, C.byref(tu),
-                                    C.byref(ms), C.byref(n)), "pool_autotune")
+    check(load().ldpc_pool_autotune(code.handle, variant, max_iterations, stop_rule, ebn0_db, sample_frames,
+                                    C.byref(tu), C.byref(ms), C.byref(n)), "pool_autotune")
     return {"words_per_block": tu.words_per_block, "threads": tu.threads, "uniform": tu.uniform,
             "batch": tu.batch, "ms": ms.value, "tried": n.value}
 
@@ -425,7 +426,8 @@ class MixedDecoder:
     """One decode step over frames of several codes (one frame-queue decoder
     per code, side by side on the GPU, optionally replayed from a hipGraph)."""
 
-    def __init__(self, codes, variant: int, max_iterations: int, tunings=None, max_frames: int = 0):
+    def __init__(self, codes, variant: int, max_iterations: int, tunings=None, max_frames: int = 0,
+                 stop_rule: int = 0):
         self.codes = list(codes)
         hs = (C.c_void_p * len(self.codes))(*[c.handle.value for c in self.codes])
         tarr = None
@@ -435,7 +437,7 @@ class MixedDecoder:
                 t = t or {}
                 tarr[i] = Tuning(t.get("words_per_block", 0), t.get("threads", 0), -1, -1, t.get("uniform", -1),
                                  t.get("batch", 0))
-        h = load().ldpc_mixed_create(hs, len(self.codes), variant, max_iterations, tarr, max_frames)
+        h = load().ldpc_mixed_create(hs, len(self.codes), variant, max_iterations, stop_rule, tarr, max_frames)
         if not h:
             raise LdpcError(f"ldpc_mixed_create failed: {_lib.last_error()}")
         self._h = C.c_void_p(h)

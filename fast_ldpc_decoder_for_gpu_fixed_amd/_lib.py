@@ -77,15 +77,15 @@ SIGNATURES = {
     "ldpc_trainer_destroy": (None, [C.c_void_p]),
     "ldpc_trainer_loss_grad": (C.c_int, [C.c_void_p, _f_p, C.c_void_p, _f_p, _f_p, C.c_void_p]),
     "ldpc_trainer_fit": (C.c_int, [C.c_void_p, C.POINTER(LearnConfig), _f_p, _f_p]),
-    "ldpc_pool_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.POINTER(Tuning)]),
+    "ldpc_pool_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning)]),
     "ldpc_pool_set_table": (C.c_int, [C.c_void_p, _i8_p, C.c_int, C.c_int]),
     "ldpc_pool_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ldpc_pool_simulate": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,
                                      C.POINTER(SimStats)]),
-    "ldpc_pool_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(Tuning),
-                                     C.POINTER(C.c_double), _c_int_p]),
-    "ldpc_mixed_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
-                                       C.c_uint64]),
+    "ldpc_pool_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64,
+                                     C.POINTER(Tuning), C.POINTER(C.c_double), _c_int_p]),
+    "ldpc_mixed_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(Tuning), C.c_uint64]),
     "ldpc_mixed_set_table": (C.c_int, [C.c_void_p, C.c_int, _i8_p, C.c_int, C.c_int]),
     "ldpc_mixed_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                     C.c_void_p, C.c_int, C.c_void_p]),

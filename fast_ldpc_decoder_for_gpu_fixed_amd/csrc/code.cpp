@@ -185,9 +185,15 @@ std::string gen_qc(ldpc_code &out, int mb, int nb, int z, double p, uint64_t see
     Rng rng(seed ^ 0x51C0DE5EEDull);
     const int kb = nb - mb;
     std::vector<int> base((size_t)mb * nb, -1);
+    // info columns: each base entry present with probability p, a column
+    // redrawn until it has at least two entries (a weight-1 variable only
+    // ever sees its channel value as its extrinsic input, so it can neither
+    // be corrected by its other checks nor let the parity stop test pass
+    // when its channel sign is wrong)
     for (int c = 0; c < kb; c++) {
         int cnt = 0;
-        for (int attempt = 0; attempt < 1000 && cnt == 0; attempt++) {
+        for (int attempt = 0; attempt < 1000 && cnt < 2; attempt++) {
+            cnt = 0;
             for (int r = 0; r < mb; r++) {
                 const bool present = rng.uniform() < p;
                 const int shift = (int)rng.below((uint32_t)z);
@@ -195,7 +201,11 @@ std::string gen_qc(ldpc_code &out, int mb, int nb, int z, double p, uint64_t see
                 cnt += present;
             }
         }
-        if (cnt == 0) base[(size_t)rng.below((uint32_t)mb) * nb + c] = (int)rng.below((uint32_t)z);
+        for (int r = 0; cnt < 2 && r < mb; r++)
+            if (base[(size_t)r * nb + c] < 0) {
+                base[(size_t)r * nb + c] = (int)rng.below((uint32_t)z);
+                cnt++;
+            }
     }
     // parity part: first parity column has shift 1 at the top and bottom and
     // 0 in the middle; the others form a dual diagonal of identities.

@@ -73,6 +73,7 @@ struct PoolArgs {
     int tits, tdeg;
     const uint32_t *edge_off16;   // single-degree path: packed LDS byte offsets, or null
     int uniform_deg, batch;       // degree of that path, items in flight per thread (1, 2, 4)
+    int stop_rule;                // 0: syndrome of the decisions, 1: v2c parity (reference rule)
 };
 // dynamic LDS a pool workgroup may use (its posterior block); the static
 // slot bookkeeping and the learned table take the rest of the 160 KiB

@@ -153,14 +153,15 @@ int enqueue_step(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_off, const
 extern "C" {
 
 ldpc_mixed *ldpc_mixed_create(const ldpc_code *const *codes, int n_codes, int variant, int max_iterations,
-                              const ldpc_tuning *tunings, uint64_t max_frames) {
+                              int stop_rule, const ldpc_tuning *tunings, uint64_t max_frames) {
     if (!codes || n_codes < 1 || n_codes > 255 || max_iterations < 1) {
         ldpc::set_last_error("ldpc_mixed_create: bad arguments (1..255 codes)");
         return nullptr;
     }
     ldpc_mixed *m = new ldpc_mixed();
     for (int c = 0; c < n_codes; c++) {
-        ldpc_pool *p = codes[c] ? ldpc_pool_create(codes[c], variant, max_iterations, tunings ? &tunings[c] : nullptr)
+        ldpc_pool *p = codes[c] ? ldpc_pool_create(codes[c], variant, max_iterations, stop_rule,
+                                                   tunings ? &tunings[c] : nullptr)
                                 : nullptr;
         if (!p) {
             const std::string why = codes[c] ? ldpc_last_error() : "null code";

@@ -20,6 +20,7 @@ struct ldpc_pool {
     int tits = 0, tdeg = 0;
     uint32_t *d_edge_off16 = nullptr;  // single-degree path
     int uniform_deg = 0, batch = 1;
+    int stop_rule = 0;
 };
 
 

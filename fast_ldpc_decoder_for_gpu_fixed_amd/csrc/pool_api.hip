@@ -64,6 +64,7 @@ PoolArgs pool_args(const ldpc_pool *p) {
     a.edge_off16 = p->d_edge_off16;
     a.uniform_deg = p->uniform_deg;
     a.batch = p->batch;
+    a.stop_rule = p->stop_rule;
     return a;
 }
 
@@ -85,8 +86,10 @@ hipError_t pool_enqueue(ldpc_pool *p, const PoolIO &io, int grid, hipStream_t s)
 
 extern "C" {
 
-ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations, const ldpc_tuning *tuning) {
-    if (!code || max_iterations < 1 || variant < LDPC_MS || variant > LDPC_LNMS) {
+ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations, int stop_rule,
+                            const ldpc_tuning *tuning) {
+    if (!code || max_iterations < 1 || variant < LDPC_MS || variant > LDPC_LNMS ||
+        (stop_rule != LDPC_STOP_SYNDROME && stop_rule != LDPC_STOP_V2C_PARITY)) {
         ldpc::set_last_error("ldpc_pool_create: bad arguments");
         return nullptr;
     }
@@ -142,6 +145,7 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     p->plan = ldpc::make_plan(p->code);
     p->variant = variant;
     p->max_iters = max_iterations;
+    p->stop_rule = stop_rule;
     p->P = P;
     while ((1 << p->log2P) < P) p->log2P++;
     p->threads = T;
@@ -317,8 +321,8 @@ int ldpc_pool_simulate(ldpc_pool *p, double ebn0_db, int quant, uint64_t seed, u
     return LDPC_OK;
 }
 
-int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, double ebn0_db, uint64_t sample_frames,
-                       ldpc_tuning *best, double *best_ms, int *n_tried) {
+int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, int stop_rule, double ebn0_db,
+                       uint64_t sample_frames, ldpc_tuning *best, double *best_ms, int *n_tried) {
     if (!code || !best || max_iterations < 1 || sample_frames == 0) {
         ldpc::set_last_error("ldpc_pool_autotune: bad arguments");
         return LDPC_ERR_ARG;
@@ -348,7 +352,7 @@ int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, d
                 for (int batch : {1, 2, 4}) {
                     if (!uni && batch != 1) continue;
                     ldpc_tuning tu = {P, T, -1, -1, uni, batch};
-                    ldpc_pool *pool = ldpc_pool_create(code, variant, max_iterations, &tu);
+                    ldpc_pool *pool = ldpc_pool_create(code, variant, max_iterations, stop_rule, &tu);
                     if (!pool) continue;  // this combination does not qualify
                     if (uni && !pool->uniform_deg) {
                         ldpc_pool_destroy(pool);

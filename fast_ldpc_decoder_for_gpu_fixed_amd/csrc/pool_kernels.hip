@@ -4,11 +4,15 @@
 // A workgroup keeps 4P frame slots resident: P words of 4 byte lanes, the
 // posterior of all of them in LDS ([var][P] words, as in k_decode).  It runs
 // the layered iteration on all slots at once; after each iteration a frame
-// whose checks all had an even parity of positive v2c values (the
-// reference's stop rule, src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:237-244,
-// applied to each frame on its own) or that reached the iteration limit
-// writes its decisions and iteration count, and its lane is refilled with
-// the next frame of the queue (one atomic per workgroup and iteration).  A
+// that passes its stop test or reached the iteration limit writes its
+// decisions and iteration count, and its lane is refilled with the next
+// frame of the queue (one atomic per workgroup and iteration).  Stop tests:
+//   syndrome   -- the decisions (post > 0) satisfy every check: one read-only
+//                 pass over all checks after the iteration;
+//   v2c parity -- every check had an even parity of positive v2c values
+//                 during the iteration (the reference's rule,
+//                 src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:237-244, per frame).
+// A
 // fresh lane reads zero messages in its first iteration: the per-word byte
 // mask `keep` clears that lane of every message word it loads.  Learned
 // tables are looked up per lane with the lane's own iteration index.
@@ -37,6 +41,46 @@ __device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__re
         post[vi[j]] = pn[j];
     }
     return pr;
+}
+
+// per byte of a posterior word: 0x80 if the byte is > 0 (decision 1), else 0
+__device__ __forceinline__ uint32_t positive_bytes(uint32_t x) {
+    const uint32_t nz = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;  // bit 7: byte != 0
+    return nz & ~x & 0x80808080u;
+}
+
+// syndrome of the decisions over every check, per byte (bit 7 of each byte
+// set if some check of that frame is unsatisfied), for this thread's items
+template <int UD>
+__device__ __forceinline__ uint32_t syndrome_items(const PoolArgs &a, const uint32_t *post, int c_first, int cstride,
+                                                   int lp, int w) {
+    uint32_t synd = 0;
+    if constexpr (UD > 0) {
+        constexpr int H = (UD + 1) / 2;
+        const char *pbase = reinterpret_cast<const char *>(post) + 4 * w;
+        for (int c = c_first; c < a.C; c += cstride) {
+            uint32_t off[H];
+#pragma unroll
+            for (int h = 0; h < H; h++) off[h] = a.edge_off16[(size_t)c * H + h];
+            uint32_t par = 0;
+#pragma unroll
+            for (int j = 0; j < UD; j++) {
+                const uint32_t o = (j & 1) ? (off[j >> 1] >> 16) : (off[j >> 1] & 0xFFFFu);
+                par ^= positive_bytes(*reinterpret_cast<const uint32_t *>(pbase + o));
+            }
+            synd |= par;
+        }
+    } else {
+        for (int c = c_first; c < a.C; c += cstride) {
+            const uint2 ci = a.checks[c];
+            const int D = (int)(ci.x & 0xFFu);
+            const uint32_t *ev = a.edge_var + ci.y;
+            uint32_t par = 0;
+            for (int j = 0; j < D; j++) par ^= positive_bytes(post[(ev[j] << lp) + (uint32_t)w]);
+            synd |= par;
+        }
+    }
+    return synd;
 }
 
 template <bool EQ, int DMAXC>
@@ -190,6 +234,10 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
                 __syncthreads();
             }
         }
+        if (a.stop_rule == 0) {  // syndrome of the decisions after the iteration
+            ove = syndrome_items<UD>(a, post, tid >> lp, cstride, lp, w);
+            ovo = 0;
+        }
         if (ove) atomicOr(&s_ove[w], ove);
         if (ovo) atomicOr(&s_ovo[w], ovo);
         __syncthreads();
@@ -197,8 +245,13 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
         // ---- per-frame stop decision
         if (tid < S && s_fid[tid] >= 0) {
             const int wl = tid >> 2, k = tid & 3;
-            const uint32_t src = (k & 1) ? s_ovo[wl] : s_ove[wl];
-            const bool odd = (src >> ((k & 2) ? 31 : 15)) & 1u;
+            bool odd;
+            if (a.stop_rule == 0) {
+                odd = (s_ove[wl] >> (8 * k + 7)) & 1u;
+            } else {
+                const uint32_t src = (k & 1) ? s_ovo[wl] : s_ove[wl];
+                odd = (src >> ((k & 2) ? 31 : 15)) & 1u;
+            }
             const int it = ++s_it[tid];
             s_flag[tid] = (!odd || it >= a.max_iters) ? 1 : 0;
         }

@@ -84,8 +84,9 @@ ldpc_code *ldpc_code_load_table(const char *path, int n_vars);
  * variables rejected (BASELINE configs 1/2/4). */
 ldpc_code *ldpc_code_regular(int n_vars, int dv, int dc, uint64_t seed);
 /* Quasi-cyclic: mb x nb base graph, circulant size z, dual-diagonal parity
- * part on the last mb columns, info entries present with probability p,
- * shifts uniform in [0, z-1]. */
+ * part on the last mb columns, info entries present with probability p (an
+ * info column is redrawn until it has at least two entries), shifts uniform
+ * in [0, z-1]. */
 ldpc_code *ldpc_code_qc(int mb, int nb, int z, double p, uint64_t seed);
 /* IRA staircase code: k_info info bits, n_hi of them of degree d_hi, the rest
  * of degree d_lo, k_info checks, parity degree 2 (staircase), info edges
@@ -227,14 +228,19 @@ int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, u
  * reference: it generalises the per-word early stop of
  * src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:237-244 to each frame and keeps the
  * GPU full by refilling a stopped frame's lane from a device queue) --------
- * A frame stops after the first iteration in which every check has an even
- * parity of positive v2c values, or at max_iterations; its decisions and
- * iteration count are written and its slot takes the next queued frame.
+ * A frame stops after the first iteration (the first included) that passes
+ * its stop test, or at max_iterations; its decisions and iteration count are
+ * written and its slot takes the next queued frame.  Stop tests:
+ *   LDPC_STOP_SYNDROME    the decisions (post > 0) satisfy every check;
+ *   LDPC_STOP_V2C_PARITY  every check had an even parity of positive v2c
+ *                         values during the iteration (the reference's rule,
+ *                         applied to each frame).
  * Variants: MS, OMS_V2, NMS, 2NMS, LOMS, LNMS, and OMS on codes with a single
  * degree group (the OMS first-iteration quirk is not per-frame).  Results
- * per frame equal a one-frame layered decode with that stop rule. */
+ * per frame equal a one-frame layered decode with that stop test. */
+enum { LDPC_STOP_SYNDROME = 0, LDPC_STOP_V2C_PARITY = 1 };
 typedef struct ldpc_pool ldpc_pool;
-ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations,
+ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations, int stop_rule,
                             const ldpc_tuning *tuning);
 int ldpc_pool_set_table(ldpc_pool *pool, const int8_t *table, int n_iters, int n_degs);
 /* Decode `frames` frame-major LLRs at d_in into d_out (decisions) and
@@ -256,7 +262,7 @@ int ldpc_pool_simulate(ldpc_pool *pool, double ebn0_db, int quant, uint64_t seed
  * single-degree path, items in flight per thread (1, 2, 4); writes the fastest
  * into *best (words_per_block, threads, uniform, batch set; other fields -1).
  * Learned variants are timed with a constant table. */
-int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, double ebn0_db,
+int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, int stop_rule, double ebn0_db,
                        uint64_t sample_frames, ldpc_tuning *best, double *best_ms, int *n_tried);
 /* Words per workgroup, workgroup size, resident workgroups, the degree of
  * the single-degree path (0 = generic path) and its items per thread. */
@@ -268,7 +274,8 @@ void ldpc_pool_destroy(ldpc_pool *pool);
 /* ---- mixed-code decode step (no single counterpart in the reference, whose
  * code is fixed per binary: src/matrix/code.h:1) ------------------------------
  * One frame-queue decoder per code (tunings[c], e.g. from ldpc_pool_autotune,
- * or NULL = defaults), all with the same variant and iteration limit.  A step
+ * or NULL = defaults), all with the same variant, iteration limit and stop
+ * test.  A step
  * decodes `frames` frames held in one byte buffer: frame f belongs to code
  * d_code_ids[f] and sits at d_in + d_offsets[f] (its N bytes of LLRs); its
  * decisions go to d_out + d_offsets[f] and its iteration count to d_iters[f]
@@ -279,7 +286,7 @@ void ldpc_pool_destroy(ldpc_pool *pool);
  * max_frames sizes the per-code queues (0 = on first use). */
 typedef struct ldpc_mixed ldpc_mixed;
 ldpc_mixed *ldpc_mixed_create(const ldpc_code *const *codes, int n_codes, int variant, int max_iterations,
-                              const ldpc_tuning *tunings, uint64_t max_frames);
+                              int stop_rule, const ldpc_tuning *tunings, uint64_t max_frames);
 int ldpc_mixed_set_table(ldpc_mixed *mixed, int code_index, const int8_t *table, int n_iters, int n_degs);
 int ldpc_mixed_decode(ldpc_mixed *mixed, const int8_t *d_in, const uint64_t *d_offsets, const uint8_t *d_code_ids,
                       uint64_t frames, uint8_t *d_out, int *d_iters, int use_graph, void *stream);

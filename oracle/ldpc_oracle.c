@@ -307,17 +307,28 @@ int orc_decode_mt(int N, int C, const int *deg, const uint32_t *pos, int variant
 }
 
 /* ------------------------------------------------------------------------ */
-/* per-frame stop: the stop test of CUDA_2NMS_SIMD.cu:237-244 (every check   */
-/* has an even parity of positive v2c values) applied to each byte lane on   */
-/* its own and after every iteration, the first included.  Lanes never mix  */
-/* (every operation above is per byte), so four frames share a word and a    */
-/* frame's result is its decisions and iteration count when its lane stops.  */
+/* per-frame stop, two rules, tested after every iteration (the first     */
+/* included) on each byte lane on its own:                                  */
+/*   rule 0 (syndrome): the hard decisions (post > 0, GPU_Transpose_uint8  */
+/*     .cu:29) satisfy every check;                                         */
+/*   rule 1 (v2c parity): every check had an even parity of positive v2c   */
+/*     values during the iteration (CUDA_2NMS_SIMD.cu:237-244).             */
+/* Lanes never mix (every operation above is per byte), so four frames     */
+/* share a word; a frame's result is its decisions and iteration count when */
+/* its lane stops.                                                          */
 /* ------------------------------------------------------------------------ */
 typedef struct {
     int N, C; const int *deg; const uint32_t *pos; const size_t *eoff; int variant;
     const int8_t *llr; size_t F; int max_iters; const int8_t *table; int tits, tdeg;
-    uint8_t *hard; int *iters; size_t g0, g1;
+    uint8_t *hard; int *iters; size_t g0, g1; int stop_rule;
 } fjob_t;
+
+/* per byte: parity of (post > 0) over the check's variables, 0xFF/0 masks */
+static uint32_t syndrome_word(int d, const uint32_t *vars, const uint32_t *post) {
+    uint32_t s = 0;
+    for (int j = 0; j < d; j++) s ^= orc_vcmpgts4(post[vars[j]], 0);
+    return s;
+}
 
 static void frames_range(const fjob_t *j) {
     const int N = j->N;
@@ -340,6 +351,10 @@ static void frames_range(const fjob_t *j) {
                 ov |= check_word(d, j->pos + j->eoff[c], post, msg + j->eoff[c], 1, 0, it == 0,
                                  j->variant, 0, lp);
             }
+            if (j->stop_rule == 0) {
+                ov = 0;
+                for (int c = 0; c < j->C; c++) ov |= syndrome_word(j->deg[c], j->pos + j->eoff[c], post);
+            }
             for (int k = 0; k < 4; k++) {
                 if (done >> k & 1) continue;
                 if (ub(ov, k) == 0 || it + 1 == j->max_iters) {
@@ -359,9 +374,9 @@ static void *fjob_run(void *p) { frames_range((const fjob_t *)p); return NULL; }
 
 int orc_decode_frames(int N, int C, const int *deg, const uint32_t *pos, int variant,
                       const int8_t *llr, size_t F, int max_iters, const int8_t *table,
-                      int tits, int tdeg, uint8_t *hard, int *iters, int threads) {
+                      int tits, int tdeg, uint8_t *hard, int *iters, int stop_rule, int threads) {
     size_t *eoff;
-    if (!llr || !hard || variant == ORC_OMS) return -1;
+    if (!llr || !hard || variant == ORC_OMS || (stop_rule != 0 && stop_rule != 1)) return -1;
     if (validate(N, C, deg, pos, variant, max_iters, table, tits, tdeg, &eoff)) return -1;
     size_t G = (F + 3) / 4;
     if (threads < 1) threads = 1;
@@ -370,7 +385,7 @@ int orc_decode_frames(int N, int C, const int *deg, const uint32_t *pos, int var
     fjob_t *jobs = (fjob_t *)malloc(sizeof(fjob_t) * (size_t)threads);
     for (int t = 0; t < threads; t++) {
         fjob_t j = {N, C, deg, pos, eoff, variant, llr, F, max_iters, table, tits, tdeg, hard, iters,
-                    G * (size_t)t / (size_t)threads, G * (size_t)(t + 1) / (size_t)threads};
+                    G * (size_t)t / (size_t)threads, G * (size_t)(t + 1) / (size_t)threads, stop_rule};
         jobs[t] = j;
         pthread_create(&th[t], NULL, fjob_run, &jobs[t]);
     }

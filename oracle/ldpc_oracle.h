@@ -77,14 +77,17 @@ int orc_decode_mt(int N, int C, const int *deg, const uint32_t *pos, int variant
                   int *iters_run, int threads);
 
 /* Per-frame stop (the persistent frame-queue decoder's model): each frame is
- * decoded alone (in its own byte lane) from iteration 0, with the stop test of
- * CUDA_2NMS_SIMD.cu:237-244 applied to it after every iteration, the first
- * included, up to max_iters.  llr / hard: frame-major [F][N]; iters[F] (may be
- * NULL) = iterations run.  Variant OMS is refused: its first-iteration rule
- * depends on the degree group (use OMS_V2 = OMS on a code with one group). */
+ * decoded alone (in its own byte lane) from iteration 0 and stops after the
+ * first iteration (the first included, up to max_iters) at which
+ *   stop_rule 0: its hard decisions (post > 0) satisfy every check, or
+ *   stop_rule 1: every check had an even parity of positive v2c values
+ *                (the test of CUDA_2NMS_SIMD.cu:237-244).
+ * llr / hard: frame-major [F][N]; iters[F] (may be NULL) = iterations run.
+ * Variant OMS is refused: its first-iteration rule depends on the degree
+ * group (use OMS_V2 = OMS on a code with one group). */
 int orc_decode_frames(int N, int C, const int *deg, const uint32_t *pos, int variant,
                       const int8_t *llr, size_t F, int max_iters, const int8_t *table,
-                      int tits, int tdeg, uint8_t *hard, int *iters, int threads);
+                      int tits, int tdeg, uint8_t *hard, int *iters, int stop_rule, int threads);
 
 /* ---- layouts (src/transpose/GPU_Transpose_uint8.cu:9-130) --------------- */
 /* frame-major int8 in[(4w+k)*N + v]  ->  post[v*W + w] byte k               */

@@ -61,7 +61,7 @@ def lib() -> C.CDLL:
         L.orc_shadow_forward.argtypes = L.orc_shadow_loss_grad.argtypes
         L.orc_decode_frames.restype = C.c_int
         L.orc_decode_frames.argtypes = [C.c_int, C.c_int, _i_p, _u32_p, C.c_int, _i8_p, C.c_size_t, C.c_int, _i8_p,
-                                        C.c_int, C.c_int, _u8_p, _i_p, C.c_int]
+                                        C.c_int, C.c_int, _u8_p, _i_p, C.c_int, C.c_int]
         L.orc_syndrome.restype = None
         L.orc_syndrome.argtypes = [_u8_p, C.c_int, C.c_int, _i_p, _u32_p, C.c_size_t, _u8_p]
         _lib = L
@@ -92,8 +92,11 @@ def deinterleave_hd(post: np.ndarray, n: int) -> np.ndarray:
     return out
 
 
+STOP_SYNDROME, STOP_V2C_PARITY = 0, 1
+
+
 def decode_frames(deg, pos, n_vars: int, variant: int, llr: np.ndarray, max_iters: int, table=None,
-                  threads: int = 1):
+                  threads: int = 1, stop_rule: int = STOP_SYNDROME):
     """Per-frame stop decode (the frame-queue decoder's model): returns
     (hard [F][N] uint8, iterations [F] int32)."""
     deg = np.ascontiguousarray(deg, dtype=np.int32)
@@ -106,7 +109,7 @@ def decode_frames(deg, pos, n_vars: int, variant: int, llr: np.ndarray, max_iter
     targs = (_p(t, C.c_int8), t.shape[0], t.shape[1]) if t is not None else (None, 0, 0)
     rc = lib().orc_decode_frames(n_vars, deg.size, _p(deg, C.c_int), _p(pos, C.c_uint32), variant,
                                  _p(llr, C.c_int8), F, max_iters, *targs, _p(hard, C.c_uint8), _p(its, C.c_int),
-                                 threads)
+                                 stop_rule, threads)
     if rc != 0:
         raise ValueError("oracle rejected the arguments")
     return hard, its

@@ -80,14 +80,15 @@ def test_qc_structure():
     c = L.Code.qc(12, 24, 81, 0.35, 1)
     assert (c.n_vars, c.n_checks) == (1944, 972)
     deg, pos, vc = var_checks(c)
-    # every column-block position is connected; parity columns have degree 2 or 3
-    assert all(len(x) >= 1 for x in vc)
+    # info columns have degree >= 2; parity columns have degree 2 or 3
+    assert all(len(x) >= 2 for x in vc)
     assert Counter(len(vc[v]) for v in range(12 * 81 + 81, 24 * 81)) == {2: 11 * 81}
     assert {len(vc[v]) for v in range(12 * 81, 13 * 81)} == {3}
     # checks of one base row never share a variable: levels <= 12
     assert c.n_levels <= 12
     c2 = L.Code.qc(4, 24, 96, 0.35, 2)
     assert (c2.n_vars, c2.n_checks) == (2304, 384) and c2.n_info == 1920
+    assert all(len(x) >= 2 for x in var_checks(c2)[2])
 
 
 def test_ira_structure():

@@ -32,13 +32,13 @@ def table_for(variant, iters=20, degs=33, seed=5):
     return rng.integers(lo, hi + 1, size=(iters, degs)).astype(np.int8)
 
 
-def check_pool(code, variant, llr, max_it, workgroups=0, **tuning):
+def check_pool(code, variant, llr, max_it, workgroups=0, stop_rule=L.STOP_SYNDROME, **tuning):
     deg, pos = code.export()
     table = table_for(variant)
-    pool = L.PoolDecoder(code, variant, max_it, table=table, **tuning)
+    pool = L.PoolDecoder(code, variant, max_it, table=table, stop_rule=stop_rule, **tuning)
     hard, its = pool.decode(llr, workgroups=workgroups)
     ref_h, ref_i = O.decode_frames(deg, pos, code.n_vars, L.OMS_V2 if variant == L.OMS else variant, llr, max_it,
-                                   table=table, threads=THREADS)
+                                   table=table, threads=THREADS, stop_rule=stop_rule)
     np.testing.assert_array_equal(its, ref_i, err_msg="iterations per frame")
     np.testing.assert_array_equal(hard, ref_h, err_msg="decisions")
     return its
@@ -58,12 +58,13 @@ def code(name):
     return _CODES[name]
 
 
+@pytest.mark.parametrize("stop_rule", [L.STOP_SYNDROME, L.STOP_V2C_PARITY], ids=["syndrome", "v2c"])
 @pytest.mark.parametrize("variant", [L.MS, L.OMS, L.OMS_V2, L.NMS, L.NMS2, L.LOMS, L.LNMS],
                          ids=lambda v: L.VARIANT_NAMES[v])
-def test_pool_parity_refills(variant, gpu_available):
+def test_pool_parity_refills(variant, stop_rule, gpu_available):
     c = code("c36_1000")
     llr = mixed_llr(c.n_vars, 2000, seed=variant)
-    its = check_pool(c, variant, llr, 20, workgroups=3)
+    its = check_pool(c, variant, llr, 20, workgroups=3, stop_rule=stop_rule)
     assert (its < 20).any()
 
 
@@ -74,6 +75,7 @@ def test_pool_parity_codes(name, variant, gpu_available):
     rate = 1.0 - c.n_checks / c.n_vars
     llr = mixed_llr(c.n_vars, 1500, seed=7, lo=1.5, hi=4.0, rate=rate)
     check_pool(c, variant, llr, 25, workgroups=5)
+    check_pool(c, variant, llr, 25, workgroups=5, stop_rule=L.STOP_V2C_PARITY)
 
 
 @pytest.mark.parametrize("P,T,uniform,batch", [(1, 64, 1, 1), (2, 128, 1, 2), (4, 256, 1, 4), (16, 512, 1, 2),
