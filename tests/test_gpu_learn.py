@@ -7,7 +7,9 @@
   pinned by finite differences (tests/test_shadow_oracle.py);
 * the per-frame-Eb/N0 training channel equals the oracle's;
 * Adam training lowers the held-out loss, and the quantised table is used
-  bit-exactly by the integer decoder (LOMS against its oracle model)."""
+  bit-exactly by the integer decoder (LOMS against its oracle model);
+* the committed trained table beats plain OMS on identical frames (what the
+  learned part gains over the method without it)."""
 import numpy as np
 import pytest
 
@@ -100,3 +102,29 @@ def test_simulate_matches_direct_decode(gpu_available):
     et = L.make_decoder(code, W, L.OMS, early_term=1)
     r2 = et.simulate(2.5, 4 * W, 20, seed=8, snr_idx=2)
     assert 1 < r2["avg_iterations"] < 20
+
+
+def test_committed_learned_table_beats_plain_oms(gpu_available):
+    """What the trained table gains: on identical device-channel frames at
+    1.75 dB, the learned-offset decoder has far fewer frame errors than plain
+    OMS, both at 10 fixed iterations and in the frame-queue decoder (max 20,
+    syndrome stop), and it stops earlier on average."""
+    from fast_ldpc_decoder_for_gpu_fixed_amd.learned import load_table
+    from pathlib import Path
+    variant, table = load_table(str(Path(L.__file__).parent / "tables" / "loms_3_6_n4000.json"))
+    assert variant == L.LOMS
+    code = code36()
+    F = 1 << 16
+    fixed = {}
+    for name, v in (("plain", L.OMS), ("learned", L.LOMS)):
+        dec = L.make_decoder(code, F // 4, v, early_term=0)
+        if v == L.LOMS:
+            dec.set_table(table)
+        fixed[name] = dec.simulate(1.75, F, 10, seed=31, snr_idx=7)
+    assert fixed["learned"]["frame_errors"] < 0.6 * fixed["plain"]["frame_errors"], fixed
+    pool = {}
+    for name, v in (("plain", L.OMS), ("learned", L.LOMS)):
+        p = L.PoolDecoder(code, v, 20, table=table if v == L.LOMS else None)
+        pool[name] = p.simulate(1.75, F, seed=31, snr_idx=7)
+    assert pool["learned"]["frame_errors"] < 0.5 * pool["plain"]["frame_errors"], pool
+    assert pool["learned"]["avg_iterations"] < pool["plain"]["avg_iterations"]
