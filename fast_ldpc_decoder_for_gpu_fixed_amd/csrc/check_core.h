@@ -65,7 +65,11 @@ __device__ __forceinline__ void magnitudes(int variant, u16x2 min1, u16x2 min2, 
 }
 
 // ---------------------------------------------------------------------------
-// one check of degree D on one word.
+// one check of degree D on one packed-16 half (2 frames, each in the high
+// byte of a 16-bit lane; low bytes zero on input).
+// x: posterior halves, m: message halves (ignored when first).
+// Outputs: pn = sat(t + msg) and mn = msg, both with clean low bytes where
+// consumers need them (repack reads only high bytes).
 // EQ = false: r = k1 - min(a - min1, k1 - k2).  Valid when k1 - k2 <= min2 - min1
 //   (every 1-Lipschitz magnitude map: MS, OMS, NMS, learned tables): a lane
 //   that is not the minimum has a >= min2.  The residue 0x7FFF (a saturated
@@ -73,79 +77,92 @@ __device__ __forceinline__ void magnitudes(int variant, u16x2 min1, u16x2 min2, 
 //   the reference's lane does, or on k1 == k2 when min1 == min2 == 127.
 // EQ = true (2NMS, k1 - k2 can exceed min2 - min1): t is cleaned to its high
 //   byte and the test a == min1 is made explicit.
-// Returns the parity-of-positives bits of the even (x) and odd (y) frames at
-// 0x80008000: frame 0 = x bit 15, 1 = y bit 15, 2 = x bit 31, 3 = y bit 31.
+// Returns the parity-of-positives bits at 0x80008000 (bit 15: the half's
+// first frame, bit 31: its second).
 // ---------------------------------------------------------------------------
+template <int D, bool EQ>
+__device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 (&m)[D], bool first, int variant,
+                                               bool nocap, u16x2 lp, u16x2 (&pn)[D], u16x2 (&mn)[D]) {
+    constexpr bool KEEP = D <= 12;  // keep |t| and -t live across the two passes
+    u16x2 t[D], ng[KEEP ? D : 1], ab[KEEP ? D : 1];
+    u16x2 m1 = splat(0x7F00u), m2 = m1;
+    u16x2 sg = splat(0u);
+    const u16x2 zero = splat(0u);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        u16x2 v = x[j];
+        if (!first) v = sub_sat_s(v, m[j]);
+        if (EQ) v &= splat(0xFF00u);
+        t[j] = v;
+        const u16x2 y = sub_sat_s(zero, v);  // -t, top bit = (t > 0)
+        const u16x2 a = umin(v, y);          // |t| as uint
+        sg ^= y;
+        m2 = umin(m2, umax(a, m1));
+        m1 = umin(m1, a);
+        if constexpr (KEEP) {
+            ng[j] = y;
+            ab[j] = a;
+        }
+    }
+    const uint32_t parity = w32(sg) & 0x80008000u;
+
+    u16x2 k1, k2;
+    magnitudes(variant, m1, m2, nocap, lp, k1, k2);
+    const u16x2 K = k1 - k2;
+    // k1 - min(a - min1, k1 - k2) == max(sat_u(k1 + min1 - a), k2): one op
+    // less per edge (k1 + min1 <= 0xFE00 cannot wrap)
+    const u16x2 C1 = k1 + m1;
+    sg ^= splat(0x8000u);  // top bit now: parity of the others is even -> negate
+
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        u16x2 y, a;
+        if constexpr (KEEP) {
+            y = ng[j];
+            a = ab[j];
+        } else {
+            y = sub_sat_s(zero, t[j]);
+            a = umin(t[j], y);
+        }
+        u16x2 r;
+        if (EQ) {
+            const u16x2 d = a - m1;  // >= 0, zero exactly on the minimum
+            r = k2 + sub_sat_u(splat(0x0001u), d) * K;
+        } else {
+            r = umax(sub_sat_u(C1, a), k2);
+        }
+        const u16x2 sj = sign_mask(sg ^ y);
+        mn[j] = (r ^ sj) - sj;
+        pn[j] = add_sat_s(t[j], mn[j]);
+    }
+    return parity;
+}
+
+// one check of degree D on one word (4 frames = even + odd half).
+// Returns the parity bits of the even (x) and odd (y) halves at 0x80008000:
+// frame 0 = x bit 15, 1 = y bit 15, 2 = x bit 31, 3 = y bit 31.
 template <int D, bool EQ>
 __device__ __forceinline__ uint2 check_core(const uint32_t (&pw)[D], const uint32_t (&mw)[D], bool first,
                                             int variant, bool nocap, u16x2 lpe, u16x2 lpo, uint32_t (&pn)[D],
                                             uint32_t (&mn)[D]) {
-    constexpr bool KEEP = D <= 12;  // keep |t| and -t live across the two passes
-    u16x2 te[D], to[D], ne[KEEP ? D : 1], no[KEEP ? D : 1], ae[KEEP ? D : 1], ao[KEEP ? D : 1];
-    u16x2 m1e = splat(0x7F00u), m2e = m1e, m1o = m1e, m2o = m1e;
-    u16x2 se = splat(0u), so = se;
-    const u16x2 zero = splat(0u);
+    u16x2 xe[D], xo[D], me[D], mo[D], pe[D], po[D], ne[D], no[D];
 #pragma unroll
     for (int j = 0; j < D; j++) {
-        u16x2 xe = unpack_even(pw[j]), xo = unpack_odd(pw[j]);
+        xe[j] = unpack_even(pw[j]);
+        xo[j] = unpack_odd(pw[j]);
         if (!first) {
-            xe = sub_sat_s(xe, unpack_even(mw[j]));
-            xo = sub_sat_s(xo, unpack_odd(mw[j]));
-        }
-        if (EQ) {
-            xe &= splat(0xFF00u);
-            xo &= splat(0xFF00u);
-        }
-        te[j] = xe;
-        to[j] = xo;
-        const u16x2 ye = sub_sat_s(zero, xe), yo = sub_sat_s(zero, xo);  // -t, top bit = (t > 0)
-        const u16x2 be = umin(xe, ye), bo = umin(xo, yo);              // |t| as uint
-        se ^= ye;
-        so ^= yo;
-        m2e = umin(m2e, umax(be, m1e));
-        m1e = umin(m1e, be);
-        m2o = umin(m2o, umax(bo, m1o));
-        m1o = umin(m1o, bo);
-        if constexpr (KEEP) {
-            ne[j] = ye; no[j] = yo; ae[j] = be; ao[j] = bo;
+            me[j] = unpack_even(mw[j]);
+            mo[j] = unpack_odd(mw[j]);
         }
     }
-    const uint2 parity = make_uint2(w32(se) & 0x80008000u, w32(so) & 0x80008000u);
-
-    u16x2 k1e, k2e, k1o, k2o;
-    magnitudes(variant, m1e, m2e, nocap, lpe, k1e, k2e);
-    magnitudes(variant, m1o, m2o, nocap, lpo, k1o, k2o);
-    const u16x2 Ke = k1e - k2e, Ko = k1o - k2o;
-    // k1 - min(a - min1, k1 - k2) == max(sat_u(k1 + min1 - a), k2): one op
-    // less per half and edge (k1 + min1 <= 0xFE00 cannot wrap)
-    const u16x2 C1e = k1e + m1e, C1o = k1o + m1o;
-    se ^= splat(0x8000u);  // top bit now: parity of the others is even -> negate
-    so ^= splat(0x8000u);
-
+    const uint32_t pe_bits = check_half<D, EQ>(xe, me, first, variant, nocap, lpe, pe, ne);
+    const uint32_t po_bits = check_half<D, EQ>(xo, mo, first, variant, nocap, lpo, po, no);
 #pragma unroll
     for (int j = 0; j < D; j++) {
-        u16x2 ye, yo, be, bo;
-        if constexpr (KEEP) {
-            ye = ne[j]; yo = no[j]; be = ae[j]; bo = ao[j];
-        } else {
-            ye = sub_sat_s(zero, te[j]); yo = sub_sat_s(zero, to[j]);
-            be = umin(te[j], ye); bo = umin(to[j], yo);
-        }
-        u16x2 re, ro;
-        if (EQ) {
-            const u16x2 de = be - m1e, dd = bo - m1o;  // >= 0, zero exactly on the minimum
-            re = k2e + sub_sat_u(splat(0x0001u), de) * Ke;
-            ro = k2o + sub_sat_u(splat(0x0001u), dd) * Ko;
-        } else {
-            re = umax(sub_sat_u(C1e, be), k2e);
-            ro = umax(sub_sat_u(C1o, bo), k2o);
-        }
-        const u16x2 se_j = sign_mask(se ^ ye), so_j = sign_mask(so ^ yo);
-        const u16x2 me = (re ^ se_j) - se_j, mo = (ro ^ so_j) - so_j;
-        mn[j] = repack(me, mo);
-        pn[j] = repack(add_sat_s(te[j], me), add_sat_s(to[j], mo));
+        mn[j] = repack(ne[j], no[j]);
+        pn[j] = repack(pe[j], po[j]);
     }
-    return parity;
+    return make_uint2(pe_bits, po_bits);
 }
 
 // Load / store the D message words of one lane (8-byte aligned runs when D is even).
