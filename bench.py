@@ -13,6 +13,13 @@ FERs come from the device error counters on the same frames.
 --mode plain: the drop-in decoder alone on 2^16 frames, 20 fixed iterations
 (the plain-OMS HBM roofline line).
 
+--mode ira: the long IRA code (N=64800, K=32400, info degrees 8/3, staircase
+parity), 8192 frames with per-frame Eb/N0 uniform in [1.0, 1.6] dB, the
+reference quantizer trunc(8y) clamped to +-31, OMS (clamp 31, beta 1), at most
+50 iterations: the frame-queue decoder (pair mode: the 4-frame posterior block,
+259 KB, does not fit LDS) with the committed learned table and with plain OMS,
+against the drop-in decoder at 50 fixed iterations.
+
 --mode mixed: the mixed-code step on two quasi-cyclic codes (12x24 base graph,
 Z=81, N=1944, rate 1/2; 4x24, Z=96, N=2304, rate 5/6), 2^17 frames interleaved
 50/50, per-frame Eb/N0 uniform in [1.5, 3.5] dB, OMS (clamp 31, beta 1), at
@@ -46,7 +53,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", choices=["matched", "plain", "mixed"], default="matched")
+    ap.add_argument("--mode", choices=["matched", "plain", "mixed", "ira"], default="matched")
     ap.add_argument("--variant", default="OMS", help="plain mode: MS, OMS, OMS_V2, NMS, 2NMS")
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--frames", type=int, default=0, help="default: 2^20 (matched), 2^16 (plain)")
@@ -253,6 +260,57 @@ def run_plain(L, args, t_start):
     return out
 
 
+def run_ira(L, args, t_start):
+    from fast_ldpc_decoder_for_gpu_fixed_amd.learned import ira_codes, load_table
+    code = ira_codes()[1]
+    N, E, K = code.n_vars, code.n_edges, code.n_info
+    frames = args.frames or 8192
+    max_it = 50
+    variant, table = load_table(str(ROOT / "fast_ldpc_decoder_for_gpu_fixed_amd" / "tables" / "loms_ira_n64800.json"))
+    d_llr = L.DeviceBuffer(frames * N)
+    L.channel_generate_range(d_llr.ptr, N, frames, args.seed, 1.0, 1.6, code.rate, quant=1)
+    L.synchronize()
+    probe = L.make_decoder(code, 4, L.OMS_V2)
+    legs = {}
+    d_hard, d_it = L.DeviceBuffer(frames * N), L.DeviceBuffer(4 * frames)
+    for name, v, tab in (("learned_pool", variant, table), ("oms_pool", L.OMS_V2, None)):
+        pool = L.PoolDecoder(code, v, max_it, table=tab, words_per_block=args.words_per_block,
+                             threads=args.threads or 1024)
+        ms = timed(L, pool.stream, args.steps, args.warmup,
+                   lambda: pool.decode_device(d_llr.ptr, frames, d_hard.ptr, d_it.ptr))
+        its = d_it.download(np.empty(frames, np.int32)).astype(np.int64)
+        r = counts(L, probe, d_hard.ptr, frames, K)
+        alg = int(its.sum()) * (2 * E + 2 * N) + frames * 2 * N
+        r.update({"info_gbps": round(frames * K / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 4),
+                  "avg_iterations": round(float(its.mean()), 4), "roofline": roofline(alg, ms, None)})
+        r.update(pool.config())
+        legs[name] = r
+    dec = L.make_decoder(code, frames // 4, L.OMS_V2, early_term=0)
+    ms_f = timed(L, dec.stream, max(1, args.steps // 5), 1, lambda: dec.decode_device(d_llr.ptr, d_hard.ptr, max_it))
+    r = counts(L, dec, d_hard.ptr, frames, K)
+    r.update({"info_gbps": round(frames * K / (ms_f * 1e-3) / 1e9, 3), "ms_per_step": round(ms_f, 4),
+              "avg_iterations": max_it, "decoder": "drop-in, global-memory posterior, fixed iterations",
+              "roofline": roofline(frames * (max_it * (2 * E + 2 * N) + 2 * N), ms_f, None)})
+    legs["oms50_fixed"] = r
+    lead = legs["learned_pool"]
+    out = {
+        "metric": "info Gbit/s, IRA N=64800 frame-queue decode (pair mode), learned vs plain OMS, 1 MI355X",
+        "value": lead["info_gbps"], "unit": "info Gbit/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": lead["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8x2 (2 frames per packed-16 lane pair)",
+        "data": "synthetic: all-zero codeword, BPSK, device Philox AWGN, per-frame Eb/N0 U[1.0, 1.6] dB, "
+                "LLR trunc(8y) clamped to +-31",
+        "config": {"workload": "IRA N=64800 K=32400 (info degrees 8/3, staircase), %d frames, OMS clamp 31, max %d "
+                               "iterations, per-frame syndrome stop" % (frames, max_it),
+                   "edges": E, "levels": code.n_levels, "legs": legs,
+                   "speedup_vs_fixed": round(lead["info_gbps"] / legs["oms50_fixed"]["info_gbps"], 3),
+                   "parallelism": "single GPU"},
+        "roofline": lead["roofline"],
+    }
+    out["wall_s"] = round(time.perf_counter() - t_start, 3)
+    return out
+
+
 def run_mixed(L, args, t_start):
     codes = [L.Code.qc(12, 24, 81, 0.35, seed=1), L.Code.qc(4, 24, 96, 0.35, seed=2)]
     names = ["qc_12x24_z81", "qc_4x24_z96"]
@@ -347,7 +405,7 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_calibration}))
         return
     t0 = time.perf_counter()
-    run = {"matched": run_matched, "plain": run_plain, "mixed": run_mixed}[args.mode]
+    run = {"matched": run_matched, "plain": run_plain, "mixed": run_mixed, "ira": run_ira}[args.mode]
     out = run(L, args, t0)
     print(json.dumps(out), flush=True)
 

@@ -346,8 +346,11 @@ class PoolDecoder:
     device queue.  Inputs and outputs stay on the device (frame-major)."""
 
     def __init__(self, code: Code, variant: int, max_iterations: int, words_per_block: int = 0,
-                 threads: int = 0, table=None, uniform: int = -1, batch: int = 0, stop_rule: int = 0):
-        tu = Tuning(words_per_block, threads, -1, -1, uniform, batch)
+                 threads: int = 0, table=None, uniform: int = -1, batch: int = 0, stop_rule: int = 0,
+                 pairs: int = -1):
+        """pairs: -1 auto (pairs of frames only when a 4-frame word block does
+        not fit LDS), 0 words of 4 frames, 1 pairs of frames."""
+        tu = Tuning(words_per_block, threads, -1, {-1: -1, 0: 1, 1: 2}[pairs], uniform, batch)
         h = load().ldpc_pool_create(code.handle, variant, max_iterations, stop_rule, C.byref(tu))
         if not h:
             raise LdpcError(f"ldpc_pool_create failed: {_lib.last_error()}")
@@ -404,11 +407,11 @@ class PoolDecoder:
         return int(load().ldpc_pool_stream(self._h) or 0)
 
     def config(self) -> dict:
-        P, T, G, U, B = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        check(load().ldpc_pool_config(self._h, C.byref(P), C.byref(T), C.byref(G), C.byref(U), C.byref(B)),
-              "pool_config")
+        P, T, G, U, B, Q = (C.c_int() for _ in range(6))
+        check(load().ldpc_pool_config(self._h, C.byref(P), C.byref(T), C.byref(G), C.byref(U), C.byref(B),
+                                      C.byref(Q)), "pool_config")
         return {"words_per_block": P.value, "threads": T.value, "max_workgroups": G.value, "uniform_deg": U.value,
-                "batch": B.value}
+                "batch": B.value, "pairs": Q.value}
 
 
 def pool_autotune(code: Code, variant: int, max_iterations: int, ebn0_db: float = 2.5,
@@ -419,7 +422,7 @@ def pool_autotune(code: Code, variant: int, max_iterations: int, ebn0_db: float 
     check(load().ldpc_pool_autotune(code.handle, variant, max_iterations, stop_rule, ebn0_db, sample_frames,
                                     C.byref(tu), C.byref(ms), C.byref(n)), "pool_autotune")
     return {"words_per_block": tu.words_per_block, "threads": tu.threads, "uniform": tu.uniform,
-            "batch": tu.batch, "ms": ms.value, "tried": n.value}
+            "batch": tu.batch, "pairs": 1 if tu.post_in_lds == 2 else 0, "ms": ms.value, "tried": n.value}
 
 
 class MixedDecoder:
@@ -435,8 +438,8 @@ class MixedDecoder:
             tarr = (Tuning * len(self.codes))()
             for i, t in enumerate(tunings):
                 t = t or {}
-                tarr[i] = Tuning(t.get("words_per_block", 0), t.get("threads", 0), -1, -1, t.get("uniform", -1),
-                                 t.get("batch", 0))
+                tarr[i] = Tuning(t.get("words_per_block", 0), t.get("threads", 0), -1,
+                                 {-1: -1, 0: 1, 1: 2}[t.get("pairs", -1)], t.get("uniform", -1), t.get("batch", 0))
         h = load().ldpc_mixed_create(hs, len(self.codes), variant, max_iterations, stop_rule, tarr, max_frames)
         if not h:
             raise LdpcError(f"ldpc_mixed_create failed: {_lib.last_error()}")
@@ -462,11 +465,11 @@ class MixedDecoder:
                                        int(graph), v(stream)), "mixed_decode")
 
     def config(self, code_index: int) -> dict:
-        P, T, G, U, B = (C.c_int() for _ in range(5))
+        P, T, G, U, B, Q = (C.c_int() for _ in range(6))
         check(load().ldpc_mixed_config(self._h, code_index, C.byref(P), C.byref(T), C.byref(G), C.byref(U),
-                                       C.byref(B)), "mixed_config")
+                                       C.byref(B), C.byref(Q)), "mixed_config")
         return {"words_per_block": P.value, "threads": T.value, "max_workgroups": G.value, "uniform_deg": U.value,
-                "batch": B.value}
+                "batch": B.value, "pairs": Q.value}
 
 
 def mixed_layout(codes, code_ids):

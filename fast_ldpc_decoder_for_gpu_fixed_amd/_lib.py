@@ -89,9 +89,10 @@ SIGNATURES = {
     "ldpc_mixed_set_table": (C.c_int, [C.c_void_p, C.c_int, _i8_p, C.c_int, C.c_int]),
     "ldpc_mixed_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                     C.c_void_p, C.c_int, C.c_void_p]),
-    "ldpc_mixed_config": (C.c_int, [C.c_void_p, C.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
+    "ldpc_mixed_config": (C.c_int, [C.c_void_p, C.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
+                                    _c_int_p]),
     "ldpc_mixed_destroy": (None, [C.c_void_p]),
-    "ldpc_pool_config": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
+    "ldpc_pool_config": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "ldpc_pool_stream": (C.c_void_p, [C.c_void_p]),
     "ldpc_pool_destroy": (None, [C.c_void_p]),
     "ldpc_device_alloc": (C.c_void_p, [C.c_size_t]),

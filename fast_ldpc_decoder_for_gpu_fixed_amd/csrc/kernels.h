@@ -67,13 +67,14 @@ struct PoolArgs {
     const unsigned *total_ptr;    // device count of queued frames, or null
     const unsigned *frame_list;   // queue position -> frame id, or null (identity)
     const unsigned long long *frame_off;  // frame id -> byte offset, or null (f * N)
-    uint32_t *msgs;               // [grid][E][P] per-workgroup message words
+    uint32_t *msgs;               // [grid][E][P] per-workgroup message words (pairs: u16 units)
     int max_iters, variant;
     const int8_t *ltable;         // learned table or null
     int tits, tdeg;
     const uint32_t *edge_off16;   // single-degree path: packed LDS byte offsets, or null
     int uniform_deg, batch;       // degree of that path, items in flight per thread (1, 2, 4)
     int stop_rule;                // 0: syndrome of the decisions, 1: v2c parity (reference rule)
+    int pairs;                    // 1: posterior and messages hold 2 frames per 16 bits (P = pairs)
 };
 // dynamic LDS a pool workgroup may use (its posterior block); the static
 // slot bookkeeping and the learned table take the rest of the 160 KiB

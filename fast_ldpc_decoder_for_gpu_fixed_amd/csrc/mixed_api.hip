@@ -255,12 +255,13 @@ int ldpc_mixed_decode(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_offse
 }
 
 int ldpc_mixed_config(const ldpc_mixed *m, int code_index, int *words_per_block, int *threads, int *max_workgroups,
-                      int *uniform_deg, int *batch) {
+                      int *uniform_deg, int *batch, int *pairs) {
     if (!m || code_index < 0 || code_index >= (int)m->pools.size()) {
         ldpc::set_last_error("ldpc_mixed_config: bad arguments");
         return LDPC_ERR_ARG;
     }
-    return ldpc_pool_config(m->pools[code_index], words_per_block, threads, max_workgroups, uniform_deg, batch);
+    return ldpc_pool_config(m->pools[code_index], words_per_block, threads, max_workgroups, uniform_deg, batch,
+                            pairs);
 }
 
 void ldpc_mixed_destroy(ldpc_mixed *m) {

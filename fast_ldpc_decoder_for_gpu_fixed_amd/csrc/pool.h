@@ -21,6 +21,7 @@ struct ldpc_pool {
     uint32_t *d_edge_off16 = nullptr;  // single-degree path
     int uniform_deg = 0, batch = 1;
     int stop_rule = 0;
+    int pairs = 0;  // posterior and messages as 2-frame pairs (P counts pairs)
 };
 
 

@@ -65,6 +65,7 @@ PoolArgs pool_args(const ldpc_pool *p) {
     a.uniform_deg = p->uniform_deg;
     a.batch = p->batch;
     a.stop_rule = p->stop_rule;
+    a.pairs = p->pairs;
     return a;
 }
 
@@ -109,20 +110,31 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     }
     ldpc_tuning tu = {0, 0, -1, -1, -1, 0};
     if (tuning) tu = *tuning;
-    const size_t wb = (size_t)code->N * 4;
+    // posterior format: 4-frame words, or 2-frame pairs when a word block
+    // does not fit LDS (tu.post_in_lds: -1 auto, 1 words, 2 pairs)
+    if (tu.post_in_lds != -1 && tu.post_in_lds != 1 && tu.post_in_lds != 2) {
+        ldpc::set_last_error("post_in_lds for the frame-queue decoder: -1 auto, 1 words of 4 frames, 2 pairs");
+        return nullptr;
+    }
+    const bool words_fit = (size_t)code->N * 4 <= ldpc::POOL_LDS_DYNAMIC;
+    const bool pairs = tu.post_in_lds == 2 || (tu.post_in_lds == -1 && !words_fit);
+    const size_t wb = (size_t)code->N * (pairs ? 2 : 4);  // LDS bytes per word / pair
+    const int fpw = pairs ? 2 : 4;                         // frames per word / pair
     int P = tu.words_per_block;
     if (P <= 0) {
         P = 1;
         while (P * 2 <= ldpc::MAX_P && wb * (size_t)(P * 2) <= 40 * 1024) P *= 2;
     }
     if (P < 1 || P > ldpc::MAX_P || (P & (P - 1)) || wb * (size_t)P > ldpc::POOL_LDS_DYNAMIC) {
-        ldpc::set_last_error("words_per_block must be a power of two <= 16 whose posterior block fits LDS");
+        ldpc::set_last_error("words_per_block must be a power of two <= 16 whose posterior block fits LDS "
+                             "(N * 4 * P bytes, or N * 2 * P in pair mode)");
         return nullptr;
     }
     // single-degree path: every check of degree 6..8 and LDS offsets within 16 bits
     int dmin = ldpc::MAX_DEG + 1;
     for (int x : code->deg) dmin = std::min(dmin, x);
-    const bool uni_ok = dmin == code->max_deg && ldpc::uniform_degree_supported(dmin) && wb * (size_t)P <= 65535;
+    const bool uni_ok = !pairs && dmin == code->max_deg && ldpc::uniform_degree_supported(dmin) &&
+                        wb * (size_t)P <= 65535;
     if (tu.uniform == 1 && !uni_ok) {
         ldpc::set_last_error("single-degree path requested but the code/blocking does not qualify");
         return nullptr;
@@ -134,9 +146,9 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
         return nullptr;
     }
     const int tmax = ldpc::max_threads_for_degree(code->max_deg, ud > 0);
-    int T = tu.threads > 0 ? tu.threads : std::min(256, tmax);
-    if (T % 64 || T % P || T > tmax || T < 4 * P) {
-        ldpc::set_last_error("threads must be a multiple of 64 and of words_per_block, >= 4*words_per_block, "
+    int T = tu.threads > 0 ? tu.threads : std::min(pairs ? 1024 : 256, tmax);
+    if (T % 64 || T % P || T > tmax || T < fpw * P) {
+        ldpc::set_last_error("threads must be a multiple of 64 and of words_per_block, >= the frame slots, "
                              "at most " + std::to_string(tmax) + " for this code");
         return nullptr;
     }
@@ -146,6 +158,7 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     p->variant = variant;
     p->max_iters = max_iterations;
     p->stop_rule = stop_rule;
+    p->pairs = pairs ? 1 : 0;
     p->P = P;
     while ((1 << p->log2P) < P) p->log2P++;
     p->threads = T;
@@ -182,7 +195,7 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     al(&p->d_checks, ck.size());
     al(&p->d_edge_var, (size_t)code->E);
     al(&p->d_level_beg, p->plan.level_beg.size());
-    al(&p->d_msgs, (size_t)p->max_grid * code->E * P);
+    al(&p->d_msgs, ((size_t)p->max_grid * code->E * P * wb / code->N + 3) / 4);  // u32 units
     al(&p->d_next, 1);
     al(&p->d_edge_off16, off16.size());
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamDefault);
@@ -233,7 +246,7 @@ int ldpc_pool_decode(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t 
         return LDPC_ERR_STATE;
     }
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
-    const uint64_t slots = 4ull * (uint64_t)p->P;
+    const uint64_t slots = (p->pairs ? 2ull : 4ull) * (uint64_t)p->P;
     int grid = workgroups > 0 ? std::min(workgroups, p->max_grid) : p->max_grid;
     grid = (int)std::min<uint64_t>((uint64_t)grid, (frames + slots - 1) / slots);
     ldpc::PoolIO io;
@@ -344,14 +357,15 @@ int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, i
     double best_t = 1e300;
     int tried = 0;
     std::string last = "no configuration could run";
+    for (int fmt = 1; fmt <= 2; fmt++)
     for (int P = 1; e == hipSuccess && P <= ldpc::MAX_P; P *= 2) {
-        if ((size_t)code->N * 4 * P > ldpc::POOL_LDS_DYNAMIC) break;
+        if ((size_t)code->N * (fmt == 1 ? 4 : 2) * P > ldpc::POOL_LDS_DYNAMIC) break;
         for (int T = 64; e == hipSuccess && T <= 1024; T *= 2) {
-            if (T % P || T < 4 * P) continue;
-            for (int uni = 1; uni >= 0; uni--) {
+            if (T % P || T < (fmt == 1 ? 4 : 2) * P) continue;
+            for (int uni = fmt == 1 ? 1 : 0; uni >= 0; uni--) {
                 for (int batch : {1, 2, 4}) {
                     if (!uni && batch != 1) continue;
-                    ldpc_tuning tu = {P, T, -1, -1, uni, batch};
+                    ldpc_tuning tu = {P, T, -1, fmt, uni, batch};
                     ldpc_pool *pool = ldpc_pool_create(code, variant, max_iterations, stop_rule, &tu);
                     if (!pool) continue;  // this combination does not qualify
                     if (uni && !pool->uniform_deg) {
@@ -407,7 +421,7 @@ int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, i
 }
 
 int ldpc_pool_config(const ldpc_pool *p, int *words_per_block, int *threads, int *max_workgroups, int *uniform_deg,
-                     int *batch) {
+                     int *batch, int *pairs) {
     if (!p) {
         ldpc::set_last_error("null pool");
         return LDPC_ERR_ARG;
@@ -417,6 +431,7 @@ int ldpc_pool_config(const ldpc_pool *p, int *words_per_block, int *threads, int
     if (max_workgroups) *max_workgroups = p->max_grid;
     if (uniform_deg) *uniform_deg = p->uniform_deg;
     if (batch) *batch = p->batch;
+    if (pairs) *pairs = p->pairs;
     return LDPC_OK;
 }
 

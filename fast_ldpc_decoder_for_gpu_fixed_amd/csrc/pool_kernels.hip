@@ -282,6 +282,212 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pair mode: for frames whose 4-frame posterior block does not fit in LDS
+// (N * 4 bytes > the LDS budget, e.g. N = 64800), the posterior holds 2
+// frames per variable in 16 bits -- exactly one packed-16 half, so the check
+// update costs the same per frame-edge -- and the messages are 16-bit pairs
+// too ([workgroup][level-order check][pair][j]).  A u16 (b0 | b1 << 8) becomes
+// the half's register {b0 << 8, b1 << 8} with one v_perm_b32 and goes back
+// with another.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u16x2 pair_to_half(uint32_t v16) { return u2(__builtin_amdgcn_perm(0u, v16, 0x010C000Cu)); }
+__device__ __forceinline__ uint16_t half_to_pair(u16x2 h) {
+    return (uint16_t)__builtin_amdgcn_perm(0u, w32(h), 0x0C0C0301u);
+}
+
+template <int D, bool EQ>
+__device__ __forceinline__ uint32_t pair_check(uint16_t *post, const uint32_t *__restrict__ ev,
+                                               uint16_t *__restrict__ mp, int lp, int w, u16x2 keep, int variant,
+                                               u16x2 lpar) {
+    uint32_t vi[D];
+    u16x2 x[D], m[D], pn[D], mn[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) vi[j] = (ev[j] << lp) + (uint32_t)w;
+#pragma unroll
+    for (int j = 0; j < D; j++) x[j] = pair_to_half(post[vi[j]]);
+#pragma unroll
+    for (int j = 0; j < D; j++) m[j] = pair_to_half(mp[j]) & keep;
+    const uint32_t par = check_half<D, EQ>(x, m, false, variant, false, lpar, pn, mn);
+#pragma unroll
+    for (int j = 0; j < D; j++) {
+        mp[j] = half_to_pair(mn[j]);
+        post[vi[j]] = half_to_pair(pn[j]);
+    }
+    return par;
+}
+
+template <bool EQ, int DMAXC>
+__device__ __forceinline__ uint32_t pair_check_any(int D, uint16_t *post, const uint32_t *__restrict__ ev,
+                                                   uint16_t *__restrict__ mp, int lp, int w, u16x2 keep, int variant,
+                                                   u16x2 lpar) {
+    switch (D) {
+#define LDPC_QCASE(d)                                                                        \
+    case d:                                                                                  \
+        if constexpr (d <= DMAXC)                                                            \
+            return pair_check<d, EQ>(post, ev, mp, lp, w, keep, variant, lpar);             \
+        else                                                                                 \
+            return 0u;
+        LDPC_QCASE(1) LDPC_QCASE(2) LDPC_QCASE(3) LDPC_QCASE(4) LDPC_QCASE(5) LDPC_QCASE(6)
+        LDPC_QCASE(7) LDPC_QCASE(8) LDPC_QCASE(9) LDPC_QCASE(10) LDPC_QCASE(11) LDPC_QCASE(12)
+        LDPC_QCASE(13) LDPC_QCASE(14) LDPC_QCASE(15) LDPC_QCASE(16) LDPC_QCASE(17) LDPC_QCASE(18)
+        LDPC_QCASE(19) LDPC_QCASE(20) LDPC_QCASE(21) LDPC_QCASE(22) LDPC_QCASE(23) LDPC_QCASE(24)
+        LDPC_QCASE(25) LDPC_QCASE(26) LDPC_QCASE(27) LDPC_QCASE(28) LDPC_QCASE(29) LDPC_QCASE(30)
+        LDPC_QCASE(31) LDPC_QCASE(32)
+#undef LDPC_QCASE
+    default: return 0u;
+    }
+}
+
+template <bool EQ, int DMAXC>
+__global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const PoolArgs a) {
+    extern __shared__ uint16_t lds_post16[];
+    __shared__ long long s_fid[POOL_SLOTS];
+    __shared__ unsigned long long s_off[POOL_SLOTS];
+    __shared__ int s_it[POOL_SLOTS];
+    __shared__ int s_flag[POOL_SLOTS];  // 1: stop this iteration, 2: refilled
+    __shared__ uint32_t s_keep[MAX_P];  // per pair: 0xFF per byte whose messages are kept
+    __shared__ uint32_t s_ov[MAX_P];
+    __shared__ int8_t s_tab[POOL_TABLE];
+    __shared__ int s_active;
+
+    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P, N = a.N;
+    const int S = 2 * P;
+    const int w = tid & (P - 1);
+    const int cstride = T >> lp;
+    uint16_t *post = lds_post16;
+    uint16_t *mbase = reinterpret_cast<uint16_t *>(a.msgs) + (size_t)blockIdx.x * a.E * P;
+    uint8_t *pb = reinterpret_cast<uint8_t *>(post);
+    const bool learned = a.ltable != nullptr;
+    if (learned)
+        for (int i = tid; i < a.tits * a.tdeg; i += T) s_tab[i] = a.ltable[i];
+    if (tid < S) {
+        s_fid[tid] = -1;
+        s_it[tid] = 0;
+        s_flag[tid] = 1;
+    }
+    if (tid < P) s_keep[tid] = 0xFFFFu;
+    __syncthreads();
+
+    for (;;) {
+        // ---- refill (as k_pool)
+        if (tid == 0) {
+            int n = 0;
+            for (int s = 0; s < S; s++) n += s_flag[s] == 1;
+            const unsigned long long total = a.total_ptr ? (unsigned long long)*a.total_ptr : a.total;
+            unsigned long long base = n ? atomicAdd(a.next, (unsigned long long)n) : 0ull;
+            int active = 0;
+            for (int s = 0; s < S; s++) {
+                if (s_flag[s] == 1) {
+                    if (base < total) {
+                        s_fid[s] = (long long)base;
+                        s_it[s] = 0;
+                        s_flag[s] = 2;
+                        s_keep[s >> 1] &= ~(0xFFu << (8 * (s & 1)));
+                    } else {
+                        s_fid[s] = -1;
+                        s_flag[s] = 0;
+                    }
+                    base++;
+                } else {
+                    s_flag[s] = 0;
+                }
+                active += s_fid[s] >= 0;
+            }
+            s_active = active;
+        }
+        __syncthreads();
+        if (s_active == 0) break;
+        if (tid < S && s_flag[tid] == 2) {
+            const unsigned long long q = (unsigned long long)s_fid[tid];
+            const unsigned long long f = a.frame_list ? (unsigned long long)a.frame_list[q] : q;
+            s_fid[tid] = (long long)f;
+            s_off[tid] = a.frame_off ? a.frame_off[f] : f * (unsigned long long)N;
+        }
+        __syncthreads();
+        for (int s = 0; s < S; s++) {
+            if (s_flag[s] != 2) continue;
+            const int8_t *src = a.in_frames + s_off[s];
+            const int wl = s >> 1, k = s & 1;
+            for (int v = tid; v < N; v += T) pb[((size_t)v * P + wl) * 2 + k] = (uint8_t)src[v];
+        }
+        if (tid < P) s_ov[tid] = 0;
+        __syncthreads();
+
+        // ---- one layered iteration over all slots
+        const u16x2 keep = pair_to_half(s_keep[w]);
+        int rowoff[2] = {0, 0};
+        if (learned)
+#pragma unroll
+            for (int k = 0; k < 2; k++) rowoff[k] = min(s_it[2 * w + k], a.tits - 1) * a.tdeg;
+        uint32_t ov = 0;
+        for (int L = 0; L < a.n_levels; L++) {
+            const int lb = a.level_beg[L], le = a.level_beg[L + 1];
+            for (int c = lb + (tid >> lp); c < le; c += cstride) {
+                const uint2 ci = a.checks[c];
+                const int D = (int)(ci.x & 0xFFu);
+                u16x2 lpar = splat(0u);
+                if (learned) {
+                    const int col = min(D, a.tdeg - 1);
+                    lpar = u2(learned_half(a.variant, s_tab[rowoff[0] + col]) |
+                              (learned_half(a.variant, s_tab[rowoff[1] + col]) << 16));
+                }
+                ov |= pair_check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y, mbase + (size_t)ci.y * P + (size_t)w * D,
+                                                lp, w, keep, a.variant, lpar);
+            }
+            __syncthreads();
+        }
+        if (a.stop_rule == 0) {  // syndrome of the decisions after the iteration
+            ov = 0;
+            for (int c = tid >> lp; c < a.C; c += cstride) {
+                const uint2 ci = a.checks[c];
+                const int D = (int)(ci.x & 0xFFu);
+                const uint32_t *ev = a.edge_var + ci.y;
+                uint32_t par = 0;
+                for (int j = 0; j < D; j++) par ^= positive_bytes(post[(ev[j] << lp) + (uint32_t)w]);
+                ov |= par;
+            }
+        }
+        if (ov) atomicOr(&s_ov[w], ov);
+        __syncthreads();
+
+        // ---- per-frame stop decision
+        if (tid < S && s_fid[tid] >= 0) {
+            const int wl = tid >> 1, k = tid & 1;
+            const uint32_t bit = a.stop_rule == 0 ? (8u * k + 7u) : (k ? 31u : 15u);
+            const bool odd = (s_ov[wl] >> bit) & 1u;
+            const int it = ++s_it[tid];
+            s_flag[tid] = (!odd || it >= a.max_iters) ? 1 : 0;
+        }
+        if (tid < P) s_keep[tid] = 0xFFFFu;
+        __syncthreads();
+
+        // ---- write the stopped frames: decisions (post > 0) and iteration count
+        for (int s = 0; s < S; s++) {
+            if (s_flag[s] != 1 || s_fid[s] < 0) continue;
+            const size_t f = (size_t)s_fid[s];
+            const int wl = s >> 1, k = s & 1;
+            uint8_t *dst = a.out_hard + s_off[s];
+            for (int v = tid; v < N; v += T) dst[v] = (uint8_t)((int8_t)pb[((size_t)v * P + wl) * 2 + k] > 0);
+            if (tid == 0 && a.out_iters) a.out_iters[f] = s_it[s];
+        }
+        __syncthreads();
+    }
+}
+
+template <bool EQ, int DMAXC>
+static hipError_t launch_pairs_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
+    hipLaunchKernelGGL((k_pool_pairs<EQ, DMAXC>), dim3(grid), dim3(threads), (size_t)a.N * a.P * 2, s, a);
+    return hipGetLastError();
+}
+
+template <bool EQ>
+static const void *pairs_kernel(int dmax) {
+    if (dmax <= 8) return (const void *)k_pool_pairs<EQ, 8>;
+    if (dmax <= 16) return (const void *)k_pool_pairs<EQ, 16>;
+    return (const void *)k_pool_pairs<EQ, 32>;
+}
+
 template <bool EQ, int DMAXC, int UD, int G>
 static hipError_t launch_pool_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
     hipLaunchKernelGGL((k_pool<EQ, DMAXC, UD, G>), dim3(grid), dim3(threads), (size_t)a.N * a.P * 4, s, a);
@@ -314,6 +520,14 @@ static hipError_t launch_pool_eq(const PoolArgs &a, int dmax, int ud, int G, int
 }
 
 hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s) {
+    if (a.pairs) {
+        if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax, false) || grid < 1 || a.uniform_deg)
+            return hipErrorInvalidValue;
+        const bool eq = a.variant == VAR_2NMS;
+        if (dmax <= 8) return eq ? launch_pairs_t<true, 8>(a, grid, threads, s) : launch_pairs_t<false, 8>(a, grid, threads, s);
+        if (dmax <= 16) return eq ? launch_pairs_t<true, 16>(a, grid, threads, s) : launch_pairs_t<false, 16>(a, grid, threads, s);
+        return eq ? launch_pairs_t<true, 32>(a, grid, threads, s) : launch_pairs_t<false, 32>(a, grid, threads, s);
+    }
     const int ud = a.uniform_deg;
     if (ud > 0 && !a.edge_off16) return hipErrorInvalidValue;
     if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax, ud > 0) || grid < 1)
@@ -326,10 +540,13 @@ hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipSt
 int pool_prepare(const PoolArgs &a, int dmax, int threads) {
     int n = 0;
     const int ud = a.uniform_deg;
-    const void *f = a.variant == VAR_2NMS ? pool_kernel<true>(dmax, ud, a.batch) : pool_kernel<false>(dmax, ud, a.batch);
+    const bool eq = a.variant == VAR_2NMS;
+    const void *f = a.pairs ? (eq ? pairs_kernel<true>(dmax) : pairs_kernel<false>(dmax))
+                            : (eq ? pool_kernel<true>(dmax, ud, a.batch) : pool_kernel<false>(dmax, ud, a.batch));
+    const size_t lds = (size_t)a.N * a.P * (a.pairs ? 2 : 4);
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)POOL_LDS_DYNAMIC) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, (size_t)a.N * a.P * 4) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, lds) != hipSuccess) return 0;
     return n;
 }
 

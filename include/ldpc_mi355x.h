@@ -237,7 +237,11 @@ int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, u
  *                         applied to each frame).
  * Variants: MS, OMS_V2, NMS, 2NMS, LOMS, LNMS, and OMS on codes with a single
  * degree group (the OMS first-iteration quirk is not per-frame).  Results
- * per frame equal a one-frame layered decode with that stop test. */
+ * per frame equal a one-frame layered decode with that stop test.
+ * Posterior format (tuning->post_in_lds): 1 = words of 4 frames (N * 4 * P
+ * bytes of LDS per workgroup), 2 = pairs of frames (N * 2 * P bytes; for
+ * long frames such as N = 64800 whose word block does not fit), -1 = words
+ * when they fit, else pairs.  words_per_block then counts words or pairs. */
 enum { LDPC_STOP_SYNDROME = 0, LDPC_STOP_V2C_PARITY = 1 };
 typedef struct ldpc_pool ldpc_pool;
 ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations, int stop_rule,
@@ -264,10 +268,11 @@ int ldpc_pool_simulate(ldpc_pool *pool, double ebn0_db, int quant, uint64_t seed
  * Learned variants are timed with a constant table. */
 int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, int stop_rule, double ebn0_db,
                        uint64_t sample_frames, ldpc_tuning *best, double *best_ms, int *n_tried);
-/* Words per workgroup, workgroup size, resident workgroups, the degree of
- * the single-degree path (0 = generic path) and its items per thread. */
+/* Words (pairs) per workgroup, workgroup size, resident workgroups, the
+ * degree of the single-degree path (0 = generic path), its items per thread,
+ * and whether the posterior holds 2-frame pairs. */
 int ldpc_pool_config(const ldpc_pool *pool, int *words_per_block, int *threads, int *max_workgroups,
-                     int *uniform_deg, int *batch);
+                     int *uniform_deg, int *batch, int *pairs);
 void *ldpc_pool_stream(ldpc_pool *pool);
 void ldpc_pool_destroy(ldpc_pool *pool);
 
@@ -291,7 +296,7 @@ int ldpc_mixed_set_table(ldpc_mixed *mixed, int code_index, const int8_t *table,
 int ldpc_mixed_decode(ldpc_mixed *mixed, const int8_t *d_in, const uint64_t *d_offsets, const uint8_t *d_code_ids,
                       uint64_t frames, uint8_t *d_out, int *d_iters, int use_graph, void *stream);
 int ldpc_mixed_config(const ldpc_mixed *mixed, int code_index, int *words_per_block, int *threads,
-                      int *max_workgroups, int *uniform_deg, int *batch);
+                      int *max_workgroups, int *uniform_deg, int *batch, int *pairs);
 void ldpc_mixed_destroy(ldpc_mixed *mixed);
 
 /* ---- learned tables: float shadow decoder trained on the GPU ----------------

@@ -149,3 +149,55 @@ def test_pool_simulate_matches_direct_decode(gpu_available):
     ref_h, ref_i = O.decode_frames(*c.export(), 4000, L.OMS_V2, llr[:200], 20, threads=THREADS)
     np.testing.assert_array_equal(its[:200], ref_i)
     np.testing.assert_array_equal(hard.reshape(F, 4000)[:200], ref_h)
+
+
+# ---------------------------------------------------------------------------
+# pair mode: 2 frames per 16-bit posterior entry (long frames whose 4-frame
+# word block does not fit LDS; forced here on short codes as well)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("stop_rule", [L.STOP_SYNDROME, L.STOP_V2C_PARITY], ids=["syndrome", "v2c"])
+@pytest.mark.parametrize("variant", [L.MS, L.OMS_V2, L.NMS, L.NMS2, L.LOMS, L.LNMS],
+                         ids=lambda v: L.VARIANT_NAMES[v])
+def test_pool_pairs_parity(variant, stop_rule, gpu_available):
+    c = code("c36_1000")
+    llr = mixed_llr(c.n_vars, 1500, seed=40 + variant)
+    pool = L.PoolDecoder(c, variant, 20, pairs=1, words_per_block=4, threads=256, table=table_for(variant),
+                         stop_rule=stop_rule)
+    assert pool.config()["pairs"] == 1 and pool.config()["uniform_deg"] == 0
+    check_pool(c, variant, llr, 20, workgroups=3, stop_rule=stop_rule, pairs=1, words_per_block=4, threads=256)
+
+
+@pytest.mark.parametrize("name", ["c36_1002", "qc_4x24_z24", "qc_12x24_z27"])
+@pytest.mark.parametrize("P,T", [(1, 64), (8, 512)])
+def test_pool_pairs_codes(name, P, T, gpu_available):
+    c = code(name)
+    if c.max_deg > 8 and T > 256:
+        T = 256
+    rate = 1.0 - c.n_checks / c.n_vars
+    llr = mixed_llr(c.n_vars, 1200, seed=8, lo=1.5, hi=4.0, rate=rate)
+    check_pool(c, L.OMS_V2, llr, 25, workgroups=0, pairs=1, words_per_block=P, threads=T)
+
+
+def ira_llr(c, frames, ebn0, seed):
+    """The reference-compatible quantizer (trunc(8y) clamped to +-31)."""
+    from oracle import oracle as O
+    return O.channel_frames(c.n_vars, 0, frames, seed, 0, O.sigma(ebn0, c.n_info / c.n_vars), 1).reshape(frames, -1)
+
+
+def test_pool_pairs_ira_small(gpu_available):
+    c = L.Code.ira(600, 240, 8, 3, 2).reorder_by_colour()
+    llr = ira_llr(c, 800, 1.5, 3)
+    for v in (L.OMS_V2, L.LOMS):
+        check_pool(c, v, llr, 30, workgroups=4, pairs=1, words_per_block=2, threads=256)
+
+
+def test_pool_ira_n64800_auto_pairs(gpu_available):
+    """The long IRA code: a 4-frame word block (259 KB) does not fit LDS, so
+    the decoder picks pair mode by itself."""
+    c = L.Code.ira(32400, 12960, 8, 3, 1).reorder_by_colour()
+    llr = ira_llr(c, 24, 1.3, 5)
+    pool = L.PoolDecoder(c, L.OMS_V2, 50)
+    cfg = pool.config()
+    assert cfg["pairs"] == 1 and cfg["words_per_block"] == 1
+    its = check_pool(c, L.OMS_V2, llr, 50)
+    assert its.max() < 50
