@@ -286,7 +286,11 @@ def run_ira(L, args, t_start):
         r.update(pool.config())
         legs[name] = r
     dec = L.make_decoder(code, frames // 4, L.OMS_V2, early_term=0)
-    ms_f = timed(L, dec.stream, max(1, args.steps // 5), 1, lambda: dec.decode_device(d_llr.ptr, d_hard.ptr, max_it))
+    d_words = L.DeviceBuffer(frames * N)  # OMS_v2 takes the device word layout (the reference's xMS convention)
+    L.interleave_device(d_llr.ptr, d_words.ptr, N, frames // 4)
+    L.synchronize()
+    ms_f = timed(L, dec.stream, max(1, args.steps // 5), 1,
+                 lambda: dec.decode_device(d_words.ptr, d_hard.ptr, max_it))
     r = counts(L, dec, d_hard.ptr, frames, K)
     r.update({"info_gbps": round(frames * K / (ms_f * 1e-3) / 1e9, 3), "ms_per_step": round(ms_f, 4),
               "avg_iterations": max_it, "decoder": "drop-in, global-memory posterior, fixed iterations",

@@ -482,6 +482,13 @@ def mixed_layout(codes, code_ids):
     return off, int(n.sum())
 
 
+def interleave_device(d_in: int, d_words: int, n_vars: int, words: int, stream: int = 0) -> None:
+    """Frame-major int8 frames -> the device word layout (word [v][w], byte k =
+    frame 4w+k), the input layout of the OMS_v2 decoder."""
+    check(load().ldpc_interleave_device(C.c_void_p(d_in), C.c_void_p(d_words), n_vars, words,
+                                        C.c_void_p(stream) if stream else None), "interleave_device")
+
+
 def channel_sigma(snr_db: float, rate: float, es_n0: bool = False) -> float:
     return float(load().ldpc_channel_sigma(snr_db, rate, int(es_n0)))
 

@@ -105,19 +105,107 @@ __device__ __forceinline__ uint2 pool_check_any(int D, uint32_t *post, const uin
     }
 }
 
+// Slot bookkeeping of one workgroup, kept by thread 0 between barriers.
+// Per round: thread 0 decides which resident frames stop (their stop test or
+// the iteration limit), writes their iteration counts, and hands the free
+// slots the next frames of the queue (one atomicAdd); then every thread
+// writes the stopped frames' decisions and loads the new frames' LLRs for
+// its own variables (the same LDS bytes, read before they are overwritten,
+// so no barrier sits between the two); then the iteration runs.
+struct SlotState {
+    long long fid[POOL_SLOTS];             // frame in the slot, -1 = empty
+    unsigned long long off[POOL_SLOTS];    // its byte offset
+    unsigned long long woff[POOL_SLOTS];   // byte offset of the frame to write out
+    int it[POOL_SLOTS];                    // iterations it has run
+    int io[POOL_SLOTS];                    // bit 0: write out, bit 1: load
+    int active;
+};
+
+// thread 0: stop decisions for resident frames (odd_bit(s) = its stop test
+// failed), iteration counts, refill from the queue; returns nothing, fills st
+template <typename OddBit>
+__device__ __forceinline__ void pool_round(const PoolArgs &a, SlotState &st, int S, OddBit odd_bit) {
+    int n_free = 0;
+    for (int s = 0; s < S; s++) {
+        st.io[s] = 0;
+        if (st.fid[s] >= 0) {
+            const int it = ++st.it[s];
+            if (!odd_bit(s) || it >= a.max_iters) {
+                if (a.out_iters) a.out_iters[st.fid[s]] = it;
+                st.io[s] = 1;
+                st.woff[s] = st.off[s];
+                st.fid[s] = -1;
+            }
+        }
+        n_free += st.fid[s] < 0;
+    }
+    const unsigned long long total = a.total_ptr ? (unsigned long long)*a.total_ptr : a.total;
+    // one atomic per round (a read of the head before it would add a second
+    // round trip to this serial path; the head may pass `total`, harmlessly)
+    unsigned long long base = n_free ? atomicAdd(a.next, (unsigned long long)n_free) : total;
+    int active = 0;
+    for (int s = 0; s < S; s++) {
+        if (st.fid[s] < 0 && base < total) {
+            const unsigned long long f = a.frame_list ? (unsigned long long)a.frame_list[base] : base;
+            st.fid[s] = (long long)f;
+            st.off[s] = a.frame_off ? a.frame_off[f] : f * (unsigned long long)a.N;
+            st.it[s] = 0;
+            st.io[s] |= 2;
+            base++;
+        }
+        active += st.fid[s] >= 0;
+    }
+    st.active = active;
+}
+
+// every thread: decisions of the frames leaving their slots, LLRs of the
+// frames entering them (byte lane k of word wl of each variable; B = bytes
+// per lane group: 4 for words, 2 for pairs)
+template <int B>
+__device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, uint8_t *pb, int S, int P) {
+    const int tid = threadIdx.x, T = blockDim.x, N = a.N;
+    for (int s = 0; s < S; s++) {
+        const int io = st.io[s];
+        if (!io) continue;  // uniform: read after a barrier
+        const int wl = s / B, k = s % B;
+        const unsigned long long wo = st.woff[s], lo = st.off[s];
+        if ((N & 3) == 0 && ((wo | lo) & 3) == 0) {
+            uint32_t *dst = reinterpret_cast<uint32_t *>(a.out_hard + wo);
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(a.in_frames + lo);
+            for (int q = tid; q < (N >> 2); q += T) {
+                uint8_t *lane = pb + ((size_t)(4 * q) * P + wl) * B + k;
+                const size_t stride = (size_t)P * B;
+                if (io & 1) {
+                    uint32_t x = 0;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) x |= (uint32_t)((int8_t)lane[r * stride] > 0) << (8 * r);
+                    dst[q] = x;
+                }
+                if (io & 2) {
+                    const uint32_t y = src[q];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) lane[r * stride] = (uint8_t)(y >> (8 * r));
+                }
+            }
+        } else {
+            for (int v = tid; v < N; v += T) {
+                uint8_t *lane = pb + ((size_t)v * P + wl) * B + k;
+                if (io & 1) a.out_hard[wo + v] = (uint8_t)((int8_t)*lane > 0);
+                if (io & 2) *lane = (uint8_t)a.in_frames[lo + v];
+            }
+        }
+    }
+}
+
 template <bool EQ, int DMAXC, int UD, int G>
 __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_pool(const PoolArgs a) {
     extern __shared__ uint32_t lds_post[];
-    __shared__ long long s_fid[POOL_SLOTS];
-    __shared__ unsigned long long s_off[POOL_SLOTS];
-    __shared__ int s_it[POOL_SLOTS];
-    __shared__ int s_flag[POOL_SLOTS];  // 1: stop this iteration, 2: refilled
+    __shared__ SlotState st;
     __shared__ uint32_t s_keep[MAX_P];
     __shared__ uint32_t s_ove[MAX_P], s_ovo[MAX_P];
     __shared__ int8_t s_tab[POOL_TABLE];
-    __shared__ int s_active;
 
-    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P, N = a.N;
+    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P;
     const int S = 4 * P;
     const int w = tid & (P - 1);
     const int cstride = T >> lp;
@@ -128,69 +216,41 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
     if (learned)
         for (int i = tid; i < a.tits * a.tdeg; i += T) s_tab[i] = a.ltable[i];
     if (tid < S) {
-        s_fid[tid] = -1;
-        s_it[tid] = 0;
-        s_flag[tid] = 1;  // every slot starts empty and asks for a frame
+        st.fid[tid] = -1;
+        st.it[tid] = 0;
     }
-    if (tid < P) s_keep[tid] = 0xFFFFFFFFu;
+    if (tid < P) s_ove[tid] = s_ovo[tid] = 0;
     __syncthreads();
 
     for (;;) {
-        // ---- refill: every slot flagged 1 takes the next frame of the queue
+        // ---- thread 0: stop decisions, iteration counts, refill
         if (tid == 0) {
-            int n = 0;
-            for (int s = 0; s < S; s++) n += s_flag[s] == 1;
-            const unsigned long long total = a.total_ptr ? (unsigned long long)*a.total_ptr : a.total;
-            unsigned long long base = n ? atomicAdd(a.next, (unsigned long long)n) : 0ull;
-            int active = 0;
-            for (int s = 0; s < S; s++) {
-                if (s_flag[s] == 1) {
-                    if (base < total) {
-                        s_fid[s] = (long long)base;  // queue position; the frame id follows below
-                        s_it[s] = 0;
-                        s_flag[s] = 2;
-                        s_keep[s >> 2] &= ~(0xFFu << (8 * (s & 3)));
-                    } else {
-                        s_fid[s] = -1;
-                        s_flag[s] = 0;
-                    }
-                    base++;
-                } else {
-                    s_flag[s] = 0;
-                }
-                active += s_fid[s] >= 0;
+            pool_round(a, st, S, [&](int s) {
+                const int wl = s >> 2, k = s & 3;
+                if (a.stop_rule == 0) return (bool)((s_ove[wl] >> (8 * k + 7)) & 1u);
+                const uint32_t src = (k & 1) ? s_ovo[wl] : s_ove[wl];
+                return (bool)((src >> ((k & 2) ? 31 : 15)) & 1u);
+            });
+            for (int q = 0; q < P; q++) {
+                uint32_t keep = 0xFFFFFFFFu;
+                for (int k = 0; k < 4; k++)
+                    if (st.io[4 * q + k] & 2) keep &= ~(0xFFu << (8 * k));
+                s_keep[q] = keep;
+                s_ove[q] = s_ovo[q] = 0;
             }
-            s_active = active;
         }
         __syncthreads();
-        if (s_active == 0) break;
-        // queue position -> frame id and its byte offset, one slot per thread
-        if (tid < S && s_flag[tid] == 2) {
-            const unsigned long long q = (unsigned long long)s_fid[tid];
-            const unsigned long long f = a.frame_list ? (unsigned long long)a.frame_list[q] : q;
-            s_fid[tid] = (long long)f;
-            s_off[tid] = a.frame_off ? a.frame_off[f] : f * (unsigned long long)N;
-        }
+        // ---- decisions out, LLRs in
+        pool_io<4>(a, st, pb, S, P);
         __syncthreads();
-        // ---- load the LLRs of the refilled frames into their lanes
-        for (int s = 0; s < S; s++) {
-            if (s_flag[s] != 2) continue;  // uniform: flags are read after the barrier
-            const int8_t *src = a.in_frames + s_off[s];
-            const int wl = s >> 2, k = s & 3;
-            for (int v = tid; v < N; v += T) pb[((size_t)v * P + wl) * 4 + k] = (uint8_t)src[v];
-        }
-        if (tid < P) {
-            s_ove[tid] = 0;
-            s_ovo[tid] = 0;
-        }
-        __syncthreads();
+        if (st.active == 0) break;
 
         // ---- one layered iteration over all slots
         const uint32_t keep = s_keep[w];
         int rowoff[4] = {0, 0, 0, 0};
         if (learned)
 #pragma unroll
-            for (int k = 0; k < 4; k++) rowoff[k] = min(s_it[4 * w + k], a.tits - 1) * a.tdeg;
+            for (int k = 0; k < 4; k++) rowoff[k] = min(st.it[4 * w + k], a.tits - 1) * a.tdeg;
         uint32_t ove = 0, ovo = 0;
         if constexpr (UD > 0) {
             u16x2 lpe = splat(0u), lpo = splat(0u);
@@ -240,44 +300,6 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
         }
         if (ove) atomicOr(&s_ove[w], ove);
         if (ovo) atomicOr(&s_ovo[w], ovo);
-        __syncthreads();
-
-        // ---- per-frame stop decision
-        if (tid < S && s_fid[tid] >= 0) {
-            const int wl = tid >> 2, k = tid & 3;
-            bool odd;
-            if (a.stop_rule == 0) {
-                odd = (s_ove[wl] >> (8 * k + 7)) & 1u;
-            } else {
-                const uint32_t src = (k & 1) ? s_ovo[wl] : s_ove[wl];
-                odd = (src >> ((k & 2) ? 31 : 15)) & 1u;
-            }
-            const int it = ++s_it[tid];
-            s_flag[tid] = (!odd || it >= a.max_iters) ? 1 : 0;
-        }
-        if (tid < P) s_keep[tid] = 0xFFFFFFFFu;
-        __syncthreads();
-
-        // ---- write the stopped frames: decisions (post > 0) and iteration count
-        for (int s = 0; s < S; s++) {
-            if (s_flag[s] != 1 || s_fid[s] < 0) continue;
-            const size_t f = (size_t)s_fid[s];
-            const int wl = s >> 2, k = s & 3;
-            uint8_t *dst = a.out_hard + s_off[s];
-            if ((N & 3) == 0 && (s_off[s] & 3) == 0) {
-                uint32_t *d32 = reinterpret_cast<uint32_t *>(dst);
-                for (int q = tid; q < (N >> 2); q += T) {
-                    uint32_t x = 0;
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        x |= (uint32_t)((int8_t)pb[((size_t)(4 * q + r) * P + wl) * 4 + k] > 0) << (8 * r);
-                    d32[q] = x;
-                }
-            } else {
-                for (int v = tid; v < N; v += T) dst[v] = (uint8_t)((int8_t)pb[((size_t)v * P + wl) * 4 + k] > 0);
-            }
-            if (tid == 0 && a.out_iters) a.out_iters[f] = s_it[s];
-        }
         __syncthreads();
     }
 }
@@ -342,16 +364,12 @@ __device__ __forceinline__ uint32_t pair_check_any(int D, uint16_t *post, const 
 template <bool EQ, int DMAXC>
 __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const PoolArgs a) {
     extern __shared__ uint16_t lds_post16[];
-    __shared__ long long s_fid[POOL_SLOTS];
-    __shared__ unsigned long long s_off[POOL_SLOTS];
-    __shared__ int s_it[POOL_SLOTS];
-    __shared__ int s_flag[POOL_SLOTS];  // 1: stop this iteration, 2: refilled
+    __shared__ SlotState st;
     __shared__ uint32_t s_keep[MAX_P];  // per pair: 0xFF per byte whose messages are kept
     __shared__ uint32_t s_ov[MAX_P];
     __shared__ int8_t s_tab[POOL_TABLE];
-    __shared__ int s_active;
 
-    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P, N = a.N;
+    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P;
     const int S = 2 * P;
     const int w = tid & (P - 1);
     const int cstride = T >> lp;
@@ -362,64 +380,39 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
     if (learned)
         for (int i = tid; i < a.tits * a.tdeg; i += T) s_tab[i] = a.ltable[i];
     if (tid < S) {
-        s_fid[tid] = -1;
-        s_it[tid] = 0;
-        s_flag[tid] = 1;
+        st.fid[tid] = -1;
+        st.it[tid] = 0;
     }
-    if (tid < P) s_keep[tid] = 0xFFFFu;
+    if (tid < P) s_ov[tid] = 0;
     __syncthreads();
 
     for (;;) {
-        // ---- refill (as k_pool)
+        // ---- thread 0: stop decisions, iteration counts, refill
         if (tid == 0) {
-            int n = 0;
-            for (int s = 0; s < S; s++) n += s_flag[s] == 1;
-            const unsigned long long total = a.total_ptr ? (unsigned long long)*a.total_ptr : a.total;
-            unsigned long long base = n ? atomicAdd(a.next, (unsigned long long)n) : 0ull;
-            int active = 0;
-            for (int s = 0; s < S; s++) {
-                if (s_flag[s] == 1) {
-                    if (base < total) {
-                        s_fid[s] = (long long)base;
-                        s_it[s] = 0;
-                        s_flag[s] = 2;
-                        s_keep[s >> 1] &= ~(0xFFu << (8 * (s & 1)));
-                    } else {
-                        s_fid[s] = -1;
-                        s_flag[s] = 0;
-                    }
-                    base++;
-                } else {
-                    s_flag[s] = 0;
-                }
-                active += s_fid[s] >= 0;
+            pool_round(a, st, S, [&](int s) {
+                const uint32_t bit = a.stop_rule == 0 ? (8u * (s & 1) + 7u) : ((s & 1) ? 31u : 15u);
+                return (bool)((s_ov[s >> 1] >> bit) & 1u);
+            });
+            for (int q = 0; q < P; q++) {
+                uint32_t keep = 0xFFFFu;
+                for (int k = 0; k < 2; k++)
+                    if (st.io[2 * q + k] & 2) keep &= ~(0xFFu << (8 * k));
+                s_keep[q] = keep;
+                s_ov[q] = 0;
             }
-            s_active = active;
         }
         __syncthreads();
-        if (s_active == 0) break;
-        if (tid < S && s_flag[tid] == 2) {
-            const unsigned long long q = (unsigned long long)s_fid[tid];
-            const unsigned long long f = a.frame_list ? (unsigned long long)a.frame_list[q] : q;
-            s_fid[tid] = (long long)f;
-            s_off[tid] = a.frame_off ? a.frame_off[f] : f * (unsigned long long)N;
-        }
+        // ---- decisions out, LLRs in
+        pool_io<2>(a, st, pb, S, P);
         __syncthreads();
-        for (int s = 0; s < S; s++) {
-            if (s_flag[s] != 2) continue;
-            const int8_t *src = a.in_frames + s_off[s];
-            const int wl = s >> 1, k = s & 1;
-            for (int v = tid; v < N; v += T) pb[((size_t)v * P + wl) * 2 + k] = (uint8_t)src[v];
-        }
-        if (tid < P) s_ov[tid] = 0;
-        __syncthreads();
+        if (st.active == 0) break;
 
         // ---- one layered iteration over all slots
         const u16x2 keep = pair_to_half(s_keep[w]);
         int rowoff[2] = {0, 0};
         if (learned)
 #pragma unroll
-            for (int k = 0; k < 2; k++) rowoff[k] = min(s_it[2 * w + k], a.tits - 1) * a.tdeg;
+            for (int k = 0; k < 2; k++) rowoff[k] = min(st.it[2 * w + k], a.tits - 1) * a.tdeg;
         uint32_t ov = 0;
         for (int L = 0; L < a.n_levels; L++) {
             const int lb = a.level_beg[L], le = a.level_beg[L + 1];
@@ -449,28 +442,6 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
             }
         }
         if (ov) atomicOr(&s_ov[w], ov);
-        __syncthreads();
-
-        // ---- per-frame stop decision
-        if (tid < S && s_fid[tid] >= 0) {
-            const int wl = tid >> 1, k = tid & 1;
-            const uint32_t bit = a.stop_rule == 0 ? (8u * k + 7u) : (k ? 31u : 15u);
-            const bool odd = (s_ov[wl] >> bit) & 1u;
-            const int it = ++s_it[tid];
-            s_flag[tid] = (!odd || it >= a.max_iters) ? 1 : 0;
-        }
-        if (tid < P) s_keep[tid] = 0xFFFFu;
-        __syncthreads();
-
-        // ---- write the stopped frames: decisions (post > 0) and iteration count
-        for (int s = 0; s < S; s++) {
-            if (s_flag[s] != 1 || s_fid[s] < 0) continue;
-            const size_t f = (size_t)s_fid[s];
-            const int wl = s >> 1, k = s & 1;
-            uint8_t *dst = a.out_hard + s_off[s];
-            for (int v = tid; v < N; v += T) dst[v] = (uint8_t)((int8_t)pb[((size_t)v * P + wl) * 2 + k] > 0);
-            if (tid == 0 && a.out_iters) a.out_iters[f] = s_it[s];
-        }
         __syncthreads();
     }
 }
