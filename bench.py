@@ -22,9 +22,9 @@ against the drop-in decoder at 50 fixed iterations.
 
 --mode mixed: the mixed-code step on two quasi-cyclic codes (12x24 base graph,
 Z=81, N=1944, rate 1/2; 4x24, Z=96, N=2304, rate 5/6), 2^17 frames interleaved
-50/50, per-frame Eb/N0 uniform in [1.5, 3.5] dB, OMS (clamp 31, beta 1), at
-most 30 iterations with per-frame stop; each code's kernel autotuned, the step
-replayed from a hipGraph.
+50/50, per-frame Eb/N0 uniform in [1.5, 3.5] dB, the reference quantizer
+trunc(8y) clamped to +-31, OMS (clamp 31, beta 1), at most 30 iterations with
+per-frame stop; each code's kernel autotuned, the step replayed from a hipGraph.
 
 Prints one JSON line: info Gbit/s, the dominant kernel's roofline (algorithmic
 bytes / kernel time from HIP events on the kernel's stream) and the CPU
@@ -334,10 +334,10 @@ def run_mixed(L, args, t_start):
     d_off.upload(off)
     d_cid.upload(cid)
     for k, c in enumerate(codes):
-        L.channel_generate_range(d_in.ptr + base[k], c.n_vars, n_c[k], args.seed + k, 1.5, 3.5, c.rate)
+        L.channel_generate_range(d_in.ptr + base[k], c.n_vars, n_c[k], args.seed + k, 1.5, 3.5, c.rate, quant=1)
     L.synchronize()
     t_tune = time.perf_counter()
-    tuned = [L.pool_autotune(c, variant, max_it, ebn0_db=2.5, sample_frames=32768) for c in codes]
+    tuned = [L.pool_autotune(c, variant, max_it, ebn0_db=2.5, sample_frames=32768, quant=1) for c in codes]
     t_tune = time.perf_counter() - t_tune
     mixed = L.MixedDecoder(codes, variant, max_it, tunings=tuned, max_frames=frames)
     step = lambda graph: mixed.decode_device(d_in.ptr, d_off.ptr, d_cid.ptr, frames, d_out.ptr, d_it.ptr,
@@ -352,7 +352,7 @@ def run_mixed(L, args, t_start):
         r.update({"code": names[k], "n": c.n_vars, "k": c.n_info, "edges": c.n_edges, "max_check_degree": c.max_deg,
                   "levels": c.n_levels, "frames": n_c[k], "avg_iterations": round(float(ik.mean()), 4),
                   "iterations_hist": np.bincount(ik, minlength=max_it + 1).tolist(),
-                  "tuning": {kk: tuned[k][kk] for kk in ("words_per_block", "threads", "uniform", "batch")},
+                  "tuning": {kk: tuned[k][kk] for kk in ("words_per_block", "threads", "uniform", "batch", "pairs")},
                   "tune_ms_per_32768_frames": round(tuned[k]["ms"], 4)})
         per_code.append(r)
         alg += int(ik.sum()) * (2 * c.n_edges + 2 * c.n_vars) + n_c[k] * 2 * c.n_vars
@@ -363,7 +363,8 @@ def run_mixed(L, args, t_start):
         "value": round(gbps, 3), "unit": "info Gbit/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int8x4 (u32 words, saturating int8 lanes)",
-        "data": "synthetic: all-zero codewords, BPSK, device Philox AWGN, per-frame Eb/N0 U[1.5, 3.5] dB",
+        "data": "synthetic: all-zero codewords, BPSK, device Philox AWGN, per-frame Eb/N0 U[1.5, 3.5] dB, "
+                "LLR trunc(8y) clamped to +-31",
         "config": {"workload": "mixed QC 12x24 Z=81 + 4x24 Z=96, %d frames interleaved 50/50, OMS clamp 31, "
                                "max %d iterations, per-frame stop, hipGraph step" % (frames, max_it),
                    "frames": frames, "codes": per_code, "eager_ms_per_step": round(ms_eager, 4),

@@ -415,11 +415,11 @@ class PoolDecoder:
 
 
 def pool_autotune(code: Code, variant: int, max_iterations: int, ebn0_db: float = 2.5,
-                  sample_frames: int = 32768, stop_rule: int = 0) -> dict:
+                  sample_frames: int = 32768, stop_rule: int = 0, quant: int = 0) -> dict:
     """Fastest frame-queue tuning for this code (words per workgroup, workgroup
     size, single-degree path and its items in flight), timed on the GPU."""
     tu, ms, n = Tuning(), C.c_double(), C.c_int()
-    check(load().ldpc_pool_autotune(code.handle, variant, max_iterations, stop_rule, ebn0_db, sample_frames,
+    check(load().ldpc_pool_autotune(code.handle, variant, max_iterations, stop_rule, ebn0_db, quant, sample_frames,
                                     C.byref(tu), C.byref(ms), C.byref(n)), "pool_autotune")
     return {"words_per_block": tu.words_per_block, "threads": tu.threads, "uniform": tu.uniform,
             "batch": tu.batch, "pairs": 1 if tu.post_in_lds == 2 else 0, "ms": ms.value, "tried": n.value}

@@ -82,7 +82,7 @@ SIGNATURES = {
     "ldpc_pool_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ldpc_pool_simulate": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,
                                      C.POINTER(SimStats)]),
-    "ldpc_pool_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64,
+    "ldpc_pool_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64,
                                      C.POINTER(Tuning), C.POINTER(C.c_double), _c_int_p]),
     "ldpc_mixed_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.POINTER(Tuning), C.c_uint64]),

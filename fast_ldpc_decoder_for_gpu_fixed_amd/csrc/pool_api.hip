@@ -335,8 +335,8 @@ int ldpc_pool_simulate(ldpc_pool *p, double ebn0_db, int quant, uint64_t seed, u
 }
 
 int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, int stop_rule, double ebn0_db,
-                       uint64_t sample_frames, ldpc_tuning *best, double *best_ms, int *n_tried) {
-    if (!code || !best || max_iterations < 1 || sample_frames == 0) {
+                       int quant, uint64_t sample_frames, ldpc_tuning *best, double *best_ms, int *n_tried) {
+    if (!code || !best || max_iterations < 1 || sample_frames == 0 || (quant != 0 && quant != 1)) {
         ldpc::set_last_error("ldpc_pool_autotune: bad arguments");
         return LDPC_ERR_ARG;
     }
@@ -351,7 +351,7 @@ int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, i
     if (e == hipSuccess) e = hipMalloc((void **)&d_it, (size_t)sample_frames * sizeof(int));
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e == hipSuccess) e = ldpc::launch_channel(d_in, 0, code->N, sample_frames, 0, 0x7475E5EEDull, 0, sigma, 0, nullptr);
+    if (e == hipSuccess) e = ldpc::launch_channel(d_in, 0, code->N, sample_frames, 0, 0x7475E5EEDull, 0, sigma, quant, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     std::vector<int8_t> table((size_t)max_iterations * (ldpc::MAX_DEG + 1), variant == LDPC_LNMS ? 6 : 1);
     double best_t = 1e300;

@@ -261,13 +261,14 @@ int ldpc_pool_decode(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8
 int ldpc_pool_simulate(ldpc_pool *pool, double ebn0_db, int quant, uint64_t seed, uint32_t snr_idx,
                        uint64_t n_frames, uint64_t batch_frames, ldpc_sim_stats *out);
 /* Autotuner (no counterpart in the reference): times the frame-queue
- * decoder on sample_frames channel frames at ebn0_db for every words-per-
+ * decoder on sample_frames channel frames at ebn0_db (quantizer `quant`, as
+ * ldpc_channel_generate; iteration counts depend on it) for every words-per-
  * workgroup P (posterior block N*P*4 bytes of LDS), workgroup size and, on the
  * single-degree path, items in flight per thread (1, 2, 4); writes the fastest
  * into *best (words_per_block, threads, uniform, batch set; other fields -1).
  * Learned variants are timed with a constant table. */
 int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, int stop_rule, double ebn0_db,
-                       uint64_t sample_frames, ldpc_tuning *best, double *best_ms, int *n_tried);
+                       int quant, uint64_t sample_frames, ldpc_tuning *best, double *best_ms, int *n_tried);
 /* Words (pairs) per workgroup, workgroup size, resident workgroups, the
  * degree of the single-degree path (0 = generic path), its items per thread,
  * and whether the posterior holds 2-frame pairs. */
