@@ -145,3 +145,76 @@ public:
     }
     double get_time_sec() { return get_time_ms() / 1000.0; }
 };
+
+// ---------------------------------------------------------------------------
+// Batched decoding with per-frame stop and slot refill (no counterpart in
+// the reference): RAII over ldpc_pool_* / ldpc_mixed_*.
+// ---------------------------------------------------------------------------
+class CFrameQueueDecoder {
+    ldpc_pool *p_;
+
+public:
+    CFrameQueueDecoder(const ldpc_code *code, int variant, int max_iterations, int stop_rule = LDPC_STOP_SYNDROME,
+                       const ldpc_tuning *tuning = nullptr)
+        : p_(ldpc_pool_create(code, variant, max_iterations, stop_rule, tuning)) {
+        if (!p_) throw ldpc::Error("ldpc_pool_create");
+    }
+    CFrameQueueDecoder(const CFrameQueueDecoder &) = delete;
+    CFrameQueueDecoder &operator=(const CFrameQueueDecoder &) = delete;
+    ~CFrameQueueDecoder() { ldpc_pool_destroy(p_); }
+
+    // the fastest tuning for a code (timed on the GPU)
+    static ldpc_tuning autotune(const ldpc_code *code, int variant, int max_iterations, double ebn0_db,
+                                int quant = 0, uint64_t sample_frames = 32768, int stop_rule = LDPC_STOP_SYNDROME) {
+        ldpc_tuning t;
+        ldpc::check(ldpc_pool_autotune(code, variant, max_iterations, stop_rule, ebn0_db, quant, sample_frames, &t,
+                                       nullptr, nullptr),
+                    "ldpc_pool_autotune");
+        return t;
+    }
+    // int8 table [n_iters][n_degs] for LDPC_LOMS / LDPC_LNMS
+    void set_table(const int8_t *table, int n_iters, int n_degs) {
+        ldpc::check(ldpc_pool_set_table(p_, table, n_iters, n_degs), "ldpc_pool_set_table");
+    }
+    // frame-major device buffers; d_iters may be null
+    void decode_device(const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters = nullptr,
+                       void *stream = nullptr, int workgroups = 0) {
+        ldpc::check(ldpc_pool_decode(p_, d_in, frames, d_out, d_iters, workgroups, stream), "ldpc_pool_decode");
+    }
+    ldpc_sim_stats simulate(double ebn0_db, uint64_t frames, uint64_t seed = 1, uint32_t snr_idx = 0, int quant = 0,
+                            uint64_t batch_frames = 0) {
+        ldpc_sim_stats s;
+        ldpc::check(ldpc_pool_simulate(p_, ebn0_db, quant, seed, snr_idx, frames, batch_frames, &s),
+                    "ldpc_pool_simulate");
+        return s;
+    }
+    void *stream() const { return ldpc_pool_stream(p_); }
+    ldpc_pool *handle() const { return p_; }
+};
+
+class CMixedDecoder {
+    ldpc_mixed *m_;
+
+public:
+    CMixedDecoder(const ldpc_code *const *codes, int n_codes, int variant, int max_iterations,
+                  const ldpc_tuning *tunings = nullptr, uint64_t max_frames = 0,
+                  int stop_rule = LDPC_STOP_SYNDROME)
+        : m_(ldpc_mixed_create(codes, n_codes, variant, max_iterations, stop_rule, tunings, max_frames)) {
+        if (!m_) throw ldpc::Error("ldpc_mixed_create");
+    }
+    CMixedDecoder(const CMixedDecoder &) = delete;
+    CMixedDecoder &operator=(const CMixedDecoder &) = delete;
+    ~CMixedDecoder() { ldpc_mixed_destroy(m_); }
+
+    void set_table(int code_index, const int8_t *table, int n_iters, int n_degs) {
+        ldpc::check(ldpc_mixed_set_table(m_, code_index, table, n_iters, n_degs), "ldpc_mixed_set_table");
+    }
+    // frame f: code d_code_ids[f], LLRs at d_in + d_offsets[f]; decisions at d_out + d_offsets[f]
+    void decode(const int8_t *d_in, const uint64_t *d_offsets, const uint8_t *d_code_ids, uint64_t frames,
+                uint8_t *d_out, int *d_iters, bool use_graph = true, void *stream = nullptr) {
+        ldpc::check(ldpc_mixed_decode(m_, d_in, d_offsets, d_code_ids, frames, d_out, d_iters, use_graph ? 1 : 0,
+                                      stream),
+                    "ldpc_mixed_decode");
+    }
+    ldpc_mixed *handle() const { return m_; }
+};
