@@ -163,3 +163,19 @@ def test_autotune_picks_a_working_configuration(gpu_available):
     for k, t in enumerate(tuned):
         cfg = mixed.config(k)
         assert cfg["words_per_block"] == t["words_per_block"] and cfg["threads"] == t["threads"]
+
+
+def test_mixed_with_pair_mode_code(gpu_available):
+    """One code in pair mode (2 frames per 16-bit entry), the others in words,
+    in one step; eager and graph both equal the oracle."""
+    cs = codes()
+    cid, off, buf = workload(cs, 1000, seed=77)
+    tunings = [{"pairs": 1, "words_per_block": 2, "threads": 256}, None, {"pairs": 0}]
+    mixed = L.MixedDecoder(cs, L.OMS_V2, 20, tunings=tunings)
+    assert mixed.config(0)["pairs"] == 1 and mixed.config(2)["pairs"] == 0
+    b = Bufs(cid, off, buf)
+    ref_h, ref_i = oracle_mixed(cs, cid, off, buf, L.OMS_V2, 20)
+    for graph in (False, True, True):
+        hard, its = b.run(mixed, graph=graph)
+        np.testing.assert_array_equal(its, ref_i)
+        np.testing.assert_array_equal(hard, ref_h)
