@@ -66,6 +66,7 @@ struct GraphKey {
 
 struct ldpc_mixed {
     std::vector<ldpc_pool *> pools;
+    std::vector<int> order;  // launch order: largest LDS block per workgroup first
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> done;
     hipEvent_t fork = nullptr;
@@ -127,12 +128,16 @@ int enqueue_step(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_off, const
                  uint8_t *d_out, int *d_iters, hipStream_t s) {
     const int nc = (int)m->pools.size();
     MHIP(ldpc::launch_zero_u32(m->d_counts, nc, s));
+    // Largest LDS footprint per workgroup first: those workgroups need the
+    // most contiguous room on a CU, and a kernel launched first fills the CUs
+    // before the next queue starts dispatching (seen on graph replays); the
+    // small-footprint kernels then fill the gaps.
     const unsigned grid = (unsigned)std::min<uint64_t>((frames + 255) / 256, 4096);
     hipLaunchKernelGGL(k_bucket, dim3(grid), dim3(256), 0, s, d_cid, (unsigned long long)frames, nc, m->d_counts,
                        m->d_lists, m->cap, d_iters);
     MHIP(hipGetLastError());
     MHIP(hipEventRecord(m->fork, s));
-    for (int c = 0; c < nc; c++) {
+    for (int c : m->order) {
         MHIP(hipStreamWaitEvent(m->streams[c], m->fork, 0));
         ldpc::PoolIO io;
         io.in = d_in;
@@ -181,6 +186,9 @@ ldpc_mixed *ldpc_mixed_create(const ldpc_code *const *codes, int n_codes, int va
         if (e == hipSuccess) m->done.push_back(ev);
     }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->fork, hipEventDisableTiming);
+    for (int c = 0; c < n_codes; c++) m->order.push_back(c);
+    auto lds = [m](int c) { return (size_t)m->pools[c]->code.N * m->pools[c]->P * (m->pools[c]->pairs ? 2 : 4); };
+    std::stable_sort(m->order.begin(), m->order.end(), [&](int x, int y) { return lds(x) > lds(y); });
     if (e == hipSuccess) e = hipMalloc((void **)&m->d_counts, (size_t)n_codes * sizeof(unsigned));
     if (e != hipSuccess) {
         ldpc::set_last_error(std::string("ldpc_mixed_create: ") + hipGetErrorString(e));
