@@ -101,18 +101,18 @@ namespace {
 
 void free_decoder(ldpc_decoder *d) {
     if (!d) return;
-    hipFree(d->d_checks);
-    hipFree(d->d_edge_var);
-    hipFree(d->d_level_beg);
-    hipFree(d->d_post);
-    hipFree(d->d_msgs);
-    hipFree(d->d_in);
-    hipFree(d->d_out);
-    hipFree(d->d_stopped);
-    hipFree(d->d_iters);
-    hipFree(d->d_edge_off16);
-    hipFree(d->d_table);
-    if (d->stream) hipStreamDestroy(d->stream);
+    (void)hipFree(d->d_checks);
+    (void)hipFree(d->d_edge_var);
+    (void)hipFree(d->d_level_beg);
+    (void)hipFree(d->d_post);
+    (void)hipFree(d->d_msgs);
+    (void)hipFree(d->d_in);
+    (void)hipFree(d->d_out);
+    (void)hipFree(d->d_stopped);
+    (void)hipFree(d->d_iters);
+    (void)hipFree(d->d_edge_off16);
+    (void)hipFree(d->d_table);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
 
@@ -399,7 +399,7 @@ int ldpc_decoder_initialize(ldpc_decoder *dec) {
 }
 
 void ldpc_decoder_destroy(ldpc_decoder *dec) {
-    if (dec) hipStreamSynchronize(dec->stream);
+    if (dec) (void)hipStreamSynchronize(dec->stream);
     free_decoder(dec);
 }
 
@@ -423,7 +423,7 @@ int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, 
     for (int i = 0; i < n_iters * n_degs; i++)
         if (table[i] < lo || table[i] > hi) return fail(LDPC_ERR_ARG, "table value out of range");
     HIP_TRY(hipStreamSynchronize(dec->stream));
-    hipFree(dec->d_table);
+    (void)hipFree(dec->d_table);
     dec->d_table = nullptr;
     HIP_TRY(dalloc(&dec->d_table, (size_t)n_iters * n_degs));
     HIP_TRY(hipMemcpy(dec->d_table, table, (size_t)n_iters * n_degs, hipMemcpyHostToDevice));
@@ -644,15 +644,15 @@ int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, u
         rc = ldpc_channel_generate(dec->d_in, dec->variant == LDPC_OMS_V2 ? 1 : 0, c.N, batch, k * batch, seed,
                                    snr_idx, sigma, quant, s);
         if (rc) break;
-        hipEventRecord(e0, s);
+        (void)hipEventRecord(e0, s);
         rc = ldpc_decoder_decode_device(dec, dec->d_in, dec->d_out, iterations, s);
-        hipEventRecord(e1, s);
+        (void)hipEventRecord(e1, s);
         if (rc) break;
         rc = ldpc_decoder_count_errors(dec, dec->d_out, batch, c.N - c.C, d_cnt, nullptr, nullptr, k > 0, s);
         if (rc) break;
         if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(LDPC_ERR_HIP, "event sync"); break; }
         float t = 0.f;
-        hipEventElapsedTime(&t, e0, e1);
+        (void)hipEventElapsedTime(&t, e0, e1);
         ms += t;
         if (dec->early_term) {
             if (hipMemcpyAsync(iters.data(), dec->d_iters, dec->W * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -665,9 +665,9 @@ int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, u
     uint64_t cnt[3] = {0, 0, 0};
     if (rc == LDPC_OK && hipMemcpy(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(LDPC_ERR_HIP, "counter copy");
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(d_cnt);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d_cnt);
     if (rc) return rc;
     out->frames = batches * batch;
     out->bit_errors = cnt[0];
@@ -688,7 +688,7 @@ void *ldpc_device_alloc(size_t bytes) {
     }
     return p;
 }
-void ldpc_device_free(void *p) { hipFree(p); }
+void ldpc_device_free(void *p) { (void)hipFree(p); }
 void *ldpc_host_alloc_pinned(size_t bytes) {
     void *p = nullptr;
     hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
@@ -698,7 +698,7 @@ void *ldpc_host_alloc_pinned(size_t bytes) {
     }
     return p;
 }
-void ldpc_host_free_pinned(void *p) { hipHostFree(p); }
+void ldpc_host_free_pinned(void *p) { (void)hipHostFree(p); }
 int ldpc_memcpy_h2d(void *dst, const void *src, size_t bytes) {
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return LDPC_OK;
@@ -732,7 +732,7 @@ ldpc_timer *ldpc_timer_create(void *stream) {
     t->stream = (hipStream_t)stream;
     if (hipEventCreate(&t->a) != hipSuccess || hipEventCreate(&t->b) != hipSuccess) {
         fail(LDPC_ERR_HIP, "hipEventCreate failed");
-        if (t->a) hipEventDestroy(t->a);
+        if (t->a) (void)hipEventDestroy(t->a);
         delete t;
         return nullptr;
     }
@@ -757,8 +757,8 @@ double ldpc_timer_elapsed_ms(ldpc_timer *t) {
 }
 void ldpc_timer_destroy(ldpc_timer *t) {
     if (!t) return;
-    hipEventDestroy(t->a);
-    hipEventDestroy(t->b);
+    (void)hipEventDestroy(t->a);
+    (void)hipEventDestroy(t->b);
     delete t;
 }
 

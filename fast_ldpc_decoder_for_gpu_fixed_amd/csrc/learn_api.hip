@@ -41,17 +41,17 @@ struct ldpc_trainer {
 
 static void trainer_free(ldpc_trainer *t) {
     if (!t) return;
-    hipFree(t->d_checks);
-    hipFree(t->d_edge_var);
-    hipFree(t->d_level_beg);
-    hipFree(t->d_msg);
-    hipFree(t->d_r);
-    hipFree(t->d_pend);
-    hipFree(t->d_params);
-    hipFree(t->d_grad);
-    hipFree(t->d_loss);
-    hipFree(t->d_llr);
-    if (t->stream) hipStreamDestroy(t->stream);
+    (void)hipFree(t->d_checks);
+    (void)hipFree(t->d_edge_var);
+    (void)hipFree(t->d_level_beg);
+    (void)hipFree(t->d_msg);
+    (void)hipFree(t->d_r);
+    (void)hipFree(t->d_pend);
+    (void)hipFree(t->d_params);
+    (void)hipFree(t->d_grad);
+    (void)hipFree(t->d_loss);
+    (void)hipFree(t->d_llr);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
     delete t;
 }
 
@@ -114,7 +114,7 @@ ldpc_trainer *ldpc_trainer_create(const ldpc_code *code, int mode, int iteration
 }
 
 void ldpc_trainer_destroy(ldpc_trainer *tr) {
-    if (tr) hipStreamSynchronize(tr->stream);
+    if (tr) (void)hipStreamSynchronize(tr->stream);
     trainer_free(tr);
 }
 

@@ -27,14 +27,14 @@ void set_last_error(const std::string &m);
 namespace {
 void pool_free(ldpc_pool *p) {
     if (!p) return;
-    hipFree(p->d_checks);
-    hipFree(p->d_edge_var);
-    hipFree(p->d_level_beg);
-    hipFree(p->d_msgs);
-    hipFree(p->d_next);
-    hipFree(p->d_table);
-    hipFree(p->d_edge_off16);
-    if (p->stream) hipStreamDestroy(p->stream);
+    (void)hipFree(p->d_checks);
+    (void)hipFree(p->d_edge_var);
+    (void)hipFree(p->d_level_beg);
+    (void)hipFree(p->d_msgs);
+    (void)hipFree(p->d_next);
+    (void)hipFree(p->d_table);
+    (void)hipFree(p->d_edge_off16);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
 
@@ -176,8 +176,11 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     }
     hipDeviceProp_t prop;
     int dev = 0;
-    hipGetDevice(&dev);
-    hipGetDeviceProperties(&prop, dev);
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        ldpc::set_last_error("ldpc_pool_create: cannot query the current device");
+        pool_free(p);
+        return nullptr;
+    }
     const ldpc::PoolArgs a = ldpc::pool_args(p);
     const int per_cu = ldpc::pool_prepare(a, code->max_deg, T);
     if (per_cu < 1) {
@@ -226,7 +229,7 @@ int ldpc_pool_set_table(ldpc_pool *p, const int8_t *table, int n_iters, int n_de
             return LDPC_ERR_ARG;
         }
     PHIP(hipStreamSynchronize(p->stream));
-    hipFree(p->d_table);
+    (void)hipFree(p->d_table);
     p->d_table = nullptr;
     PHIP(hipMalloc((void **)&p->d_table, (size_t)n_iters * n_degs));
     PHIP(hipMemcpy(p->d_table, table, (size_t)n_iters * n_degs, hipMemcpyHostToDevice));
@@ -308,7 +311,7 @@ int ldpc_pool_simulate(ldpc_pool *p, double ebn0_db, int quant, uint64_t seed, u
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) break;
         float t = 0.f;
-        hipEventElapsedTime(&t, e0, e1);
+        (void)hipEventElapsedTime(&t, e0, e1);
         ms += t;
         for (uint64_t i = 0; i < nb; i++) it_sum += its[i];
     }
@@ -318,12 +321,12 @@ int ldpc_pool_simulate(ldpc_pool *p, double ebn0_db, int quant, uint64_t seed, u
         ldpc::set_last_error(std::string("ldpc_pool_simulate: ") + hipGetErrorString(e));
         rc = LDPC_ERR_HIP;
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipFree(d_llr);
-    hipFree(d_hard);
-    hipFree(d_it);
-    hipFree(d_cnt);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_llr);
+    (void)hipFree(d_hard);
+    (void)hipFree(d_it);
+    (void)hipFree(d_cnt);
     if (rc) return rc;
     out->frames = n_frames;
     out->bit_errors = cnt[0];
@@ -377,12 +380,12 @@ int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, i
                         rc = ldpc_pool_set_table(pool, table.data(), max_iterations, ldpc::MAX_DEG + 1);
                     float t_min = 1e30f;
                     for (int rep = 0; rc == LDPC_OK && rep < 3; rep++) {
-                        hipEventRecord(e0, pool->stream);
+                        (void)hipEventRecord(e0, pool->stream);
                         rc = ldpc_pool_decode(pool, d_in, sample_frames, d_out, d_it, 0, nullptr);
-                        hipEventRecord(e1, pool->stream);
+                        (void)hipEventRecord(e1, pool->stream);
                         if (rc == LDPC_OK && hipEventSynchronize(e1) == hipSuccess) {
                             float t = 0.f;
-                            hipEventElapsedTime(&t, e0, e1);
+                            (void)hipEventElapsedTime(&t, e0, e1);
                             if (rep > 0) t_min = std::min(t_min, t);  // the first run warms up
                         }
                     }
@@ -402,11 +405,11 @@ int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, i
             }
         }
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipFree(d_in);
-    hipFree(d_out);
-    hipFree(d_it);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    (void)hipFree(d_it);
     if (e != hipSuccess) {
         ldpc::set_last_error(std::string("ldpc_pool_autotune: ") + hipGetErrorString(e));
         return LDPC_ERR_HIP;
@@ -438,7 +441,7 @@ int ldpc_pool_config(const ldpc_pool *p, int *words_per_block, int *threads, int
 void *ldpc_pool_stream(ldpc_pool *p) { return p ? (void *)p->stream : nullptr; }
 
 void ldpc_pool_destroy(ldpc_pool *p) {
-    if (p) hipStreamSynchronize(p->stream);
+    if (p) (void)hipStreamSynchronize(p->stream);
     pool_free(p);
 }
 
