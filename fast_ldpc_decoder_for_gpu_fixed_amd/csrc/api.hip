@@ -610,11 +610,7 @@ int ldpc_decoder_count_errors(ldpc_decoder *dec, const uint8_t *d_hard, size_t f
         return fail(LDPC_ERR_ARG, "ldpc_decoder_count_errors: bad arguments");
     if ((size_t)dec->code.N > ldpc::MAX_LDS_BYTES - 1024)
         return fail(LDPC_ERR_ARG, "frame longer than the LDS staging buffer");
-    static bool attr = false;
-    if (!attr) {
-        HIP_TRY(ldpc::set_count_lds_limit());
-        attr = true;
-    }
+    HIP_TRY(ldpc::set_count_lds_limit());  // once per process
     HIP_TRY(ldpc::launch_count(d_hard, dec->code.N, frames, n_info, dec->d_checks, dec->d_edge_var, dec->code.C,
                                reinterpret_cast<unsigned long long *>(d_counters), d_frame_bit_errors,
                                d_frame_syndrome_ok, accumulate == 0, (hipStream_t)stream));

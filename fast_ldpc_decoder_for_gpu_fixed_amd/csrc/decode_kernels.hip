@@ -317,14 +317,14 @@ __global__ __launch_bounds__(256) void k_deinterleave_hd(const uint32_t *__restr
 template <bool EQ, bool LDS, int DMAXC, int UD, int G>
 static hipError_t launch_decode_t(const DecodeArgs &a, int threads, hipStream_t s) {
     const size_t lds = LDS ? (size_t)a.N * a.P * 4 : 0;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_decode<EQ, LDS, DMAXC, UD, G>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(MAX_LDS_BYTES - 1024));  // static flags live in the rest
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
+    static std::once_flag flag;
+    static hipError_t attr = hipSuccess;
+    const hipError_t e = once_per_process(flag, attr, [] {
+        return hipFuncSetAttribute((const void *)k_decode<EQ, LDS, DMAXC, UD, G>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(MAX_LDS_BYTES - 1024));  // static flags live in the rest
+    });
+    if (e != hipSuccess) return e;
     const unsigned nblk = (unsigned)(a.W / a.P);
     hipLaunchKernelGGL((k_decode<EQ, LDS, DMAXC, UD, G>), dim3(nblk), dim3(threads), lds, s, a);
     return hipGetLastError();

@@ -5,7 +5,17 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
 namespace ldpc {
+
+// Runs f once per process (thread-safe) and returns its result on every call;
+// used for the kernels' LDS attribute, which is set before their first launch.
+template <typename F>
+inline hipError_t once_per_process(std::once_flag &flag, hipError_t &result, F f) {
+    std::call_once(flag, [&] { result = f(); });
+    return result;
+}
 
 // numbering shared with include/ldpc_mi355x.h
 enum Variant { VAR_MS = 0, VAR_OMS = 1, VAR_OMS_V2 = 2, VAR_NMS = 3, VAR_2NMS = 4, VAR_LOMS = 5, VAR_LNMS = 6 };

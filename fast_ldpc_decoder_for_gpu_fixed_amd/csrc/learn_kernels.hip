@@ -255,16 +255,17 @@ __global__ __launch_bounds__(256) void k_shadow_backward(const ShadowArgs a) {
 
 hipError_t launch_shadow(const ShadowArgs &a, bool backward, hipStream_t s) {
     const size_t lds = (size_t)a.N * a.FB * sizeof(float);
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_shadow_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
+    static std::once_flag flag;
+    static hipError_t attr = hipSuccess;
+    const hipError_t e = once_per_process(flag, attr, [] {
+        hipError_t r = hipFuncSetAttribute((const void *)k_shadow_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(MAX_LDS_BYTES - 2048));
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_shadow_backward, hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (r == hipSuccess)
+            r = hipFuncSetAttribute((const void *)k_shadow_backward, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(MAX_LDS_BYTES - 2048));
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+        return r;
+    });
+    if (e != hipSuccess) return e;
     const unsigned nb = (unsigned)(a.frames / a.FB);
     if (backward)
         hipLaunchKernelGGL(k_shadow_backward, dim3(nb), dim3(256), lds, s, a);

@@ -275,11 +275,7 @@ int ldpc_pool_simulate(ldpc_pool *p, double ebn0_db, int quant, uint64_t seed, u
     const uint64_t batch = std::min<uint64_t>(n_frames, batch_frames ? batch_frames : (1ull << 18));
     const double rate = (double)(c.N - c.C) / (double)c.N;
     const double sigma = ldpc_channel_sigma(ebn0_db, rate, 0);
-    static bool attr = false;
-    if (!attr) {
-        PHIP(ldpc::set_count_lds_limit());
-        attr = true;
-    }
+    PHIP(ldpc::set_count_lds_limit());  // once per process
     int8_t *d_llr = nullptr;
     uint8_t *d_hard = nullptr;
     int *d_it = nullptr;

@@ -177,8 +177,12 @@ hipError_t launch_channel_range(int8_t *out, int N, size_t frames, uint64_t f0, 
 }
 
 hipError_t set_count_lds_limit() {
-    return hipFuncSetAttribute((const void *)k_count, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)(MAX_LDS_BYTES - 1024));
+    static std::once_flag flag;
+    static hipError_t result = hipSuccess;
+    return once_per_process(flag, result, [] {
+        return hipFuncSetAttribute((const void *)k_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(MAX_LDS_BYTES - 1024));
+    });
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {

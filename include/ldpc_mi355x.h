@@ -251,7 +251,8 @@ int ldpc_pool_set_table(ldpc_pool *pool, const int8_t *table, int n_iters, int n
  * d_iters (iterations per frame, may be NULL).  workgroups = 0: as many as
  * stay resident on the GPU (capped by the work).  Asynchronous on `stream`
  * (NULL = the pool's own); the queue counter is reset with a kernel, so the
- * call may be captured in a HIP graph. */
+ * call may be captured in a HIP graph.  Calls on one pool must not overlap:
+ * they share its queue head and message buffer (use one pool per stream). */
 int ldpc_pool_decode(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters,
                      int workgroups, void *stream);
 /* One Eb/N0 point: device channel (frames 0 .. n_frames-1 of the stream
