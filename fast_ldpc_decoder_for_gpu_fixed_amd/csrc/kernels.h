@@ -86,9 +86,10 @@ struct PoolArgs {
     int stop_rule;                // 0: syndrome of the decisions, 1: v2c parity (reference rule)
     int pairs;                    // 1: posterior and messages hold 2 frames per 16 bits (P = pairs)
 };
-// dynamic LDS a pool workgroup may use (its posterior block); the static
-// slot bookkeeping and the learned table take the rest of the 160 KiB
+// largest posterior block of a frame-queue workgroup; the slot bookkeeping and
+// the learned table (< 4.4 KB) follow it in the same dynamic LDS allocation
 constexpr size_t POOL_LDS_DYNAMIC = MAX_LDS_BYTES - 8192;
+size_t pool_lds_bytes(const PoolArgs &a);
 hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s);
 // sets the LDS limit of the instantiation `a` selects (call before any launch
 // and outside graph capture) and returns how many of its workgroups a CU holds

@@ -188,7 +188,8 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
         pool_free(p);
         return nullptr;
     }
-    p->max_grid = prop.multiProcessorCount * per_cu;
+    p->n_cu = prop.multiProcessorCount;
+    p->max_grid = p->n_cu * per_cu;
     std::vector<uint2> ck((size_t)code->C);
     for (int i = 0; i < code->C; i++) ck[i] = make_uint2(p->plan.check_info[i], p->plan.check_e0[i]);
     hipError_t e = hipSuccess;
@@ -235,6 +236,14 @@ int ldpc_pool_set_table(ldpc_pool *p, const int8_t *table, int n_iters, int n_de
     PHIP(hipMemcpy(p->d_table, table, (size_t)n_iters * n_degs, hipMemcpyHostToDevice));
     p->tits = n_iters;
     p->tdeg = n_degs;
+    // the table takes LDS next to the posterior block: fewer workgroups may
+    // stay resident (the message buffer was sized for the larger count)
+    const int per_cu = ldpc::pool_prepare(ldpc::pool_args(p), p->code.max_deg, p->threads);
+    if (per_cu < 1) {
+        ldpc::set_last_error("the frame-queue kernel cannot run with this table");
+        return LDPC_ERR_ARG;
+    }
+    p->max_grid = std::min(p->max_grid, p->n_cu * per_cu);
     return LDPC_OK;
 }
 

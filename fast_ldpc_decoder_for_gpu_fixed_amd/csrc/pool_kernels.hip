@@ -22,7 +22,7 @@
 namespace ldpc {
 
 constexpr int POOL_SLOTS = 4 * MAX_P;
-constexpr int POOL_TABLE = 64 * (MAX_DEG + 1);  // learned table bytes kept in LDS
+constexpr int POOL_TABLE = 64 * (MAX_DEG + 1);  // learned table bytes at most (kept in LDS)
 
 template <int D, bool EQ>
 __device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__restrict__ ev, uint32_t *__restrict__ mp,
@@ -43,43 +43,68 @@ __device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__re
     return pr;
 }
 
-// per byte of a posterior word: 0x80 if the byte is > 0 (decision 1), else 0
-__device__ __forceinline__ uint32_t positive_bytes(uint32_t x) {
-    const uint32_t nz = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;  // bit 7: byte != 0
-    return nz & ~x & 0x80808080u;
+// per byte of a posterior word: bit 7 set if the byte is > 0 (decision 1).
+// The low 7 bits plus 0x7F carry into bit 7 exactly when they are nonzero
+// (no carry leaves the byte); & ~x clears negative bytes.  Bits 0..6 of the
+// result are junk: XOR-accumulate, then keep 0x80808080 once per check.
+__device__ __forceinline__ uint32_t positive_msb(uint32_t x) {
+    return ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) & ~x;
 }
 
 // syndrome of the decisions over every check, per byte (bit 7 of each byte
-// set if some check of that frame is unsatisfied), for this thread's items
+// set if some check of that frame is unsatisfied), for this thread's items.
+// Single-degree path with P = 2: a thread takes a whole check and reads both
+// words of each variable with one 64-bit LDS load (ov.x: word 0, ov.y: word 1).
 template <int UD>
-__device__ __forceinline__ uint32_t syndrome_items(const PoolArgs &a, const uint32_t *post, int c_first, int cstride,
-                                                   int lp, int w) {
-    uint32_t synd = 0;
+__device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_t *post, int lp, int w) {
+    const int tid = threadIdx.x, T = blockDim.x;
+    uint2 synd = make_uint2(0u, 0u);
     if constexpr (UD > 0) {
         constexpr int H = (UD + 1) / 2;
-        const char *pbase = reinterpret_cast<const char *>(post) + 4 * w;
-        for (int c = c_first; c < a.C; c += cstride) {
-            uint32_t off[H];
+        if (a.P == 2) {
+            const char *pbase = reinterpret_cast<const char *>(post);
+            for (int c = tid; c < a.C; c += T) {
+                uint32_t off[H];
 #pragma unroll
-            for (int h = 0; h < H; h++) off[h] = a.edge_off16[(size_t)c * H + h];
-            uint32_t par = 0;
+                for (int h = 0; h < H; h++) off[h] = a.edge_off16[(size_t)c * H + h];
+                uint32_t p0 = 0, p1 = 0;
 #pragma unroll
-            for (int j = 0; j < UD; j++) {
-                const uint32_t o = (j & 1) ? (off[j >> 1] >> 16) : (off[j >> 1] & 0xFFFFu);
-                par ^= positive_bytes(*reinterpret_cast<const uint32_t *>(pbase + o));
+                for (int j = 0; j < UD; j++) {
+                    const uint32_t o = (j & 1) ? (off[j >> 1] >> 16) : (off[j >> 1] & 0xFFFFu);
+                    const uint2 x = *reinterpret_cast<const uint2 *>(pbase + o);
+                    p0 ^= positive_msb(x.x);
+                    p1 ^= positive_msb(x.y);
+                }
+                synd.x |= p0;
+                synd.y |= p1;
             }
-            synd |= par;
+        } else {
+            const char *pbase = reinterpret_cast<const char *>(post) + 4 * w;
+            for (int c = tid >> lp; c < a.C; c += T >> lp) {
+                uint32_t off[H];
+#pragma unroll
+                for (int h = 0; h < H; h++) off[h] = a.edge_off16[(size_t)c * H + h];
+                uint32_t par = 0;
+#pragma unroll
+                for (int j = 0; j < UD; j++) {
+                    const uint32_t o = (j & 1) ? (off[j >> 1] >> 16) : (off[j >> 1] & 0xFFFFu);
+                    par ^= positive_msb(*reinterpret_cast<const uint32_t *>(pbase + o));
+                }
+                synd.x |= par;
+            }
         }
     } else {
-        for (int c = c_first; c < a.C; c += cstride) {
+        for (int c = tid >> lp; c < a.C; c += T >> lp) {
             const uint2 ci = a.checks[c];
             const int D = (int)(ci.x & 0xFFu);
             const uint32_t *ev = a.edge_var + ci.y;
             uint32_t par = 0;
-            for (int j = 0; j < D; j++) par ^= positive_bytes(post[(ev[j] << lp) + (uint32_t)w]);
-            synd |= par;
+            for (int j = 0; j < D; j++) par ^= positive_msb(post[(ev[j] << lp) + (uint32_t)w]);
+            synd.x |= par;
         }
     }
+    synd.x &= 0x80808080u;
+    synd.y &= 0x80808080u;
     return synd;
 }
 
@@ -113,18 +138,57 @@ __device__ __forceinline__ uint2 pool_check_any(int D, uint32_t *post, const uin
 // its own variables (the same LDS bytes, read before they are overwritten,
 // so no barrier sits between the two); then the iteration runs.
 struct SlotState {
-    long long fid[POOL_SLOTS];             // frame in the slot, -1 = empty
-    unsigned long long off[POOL_SLOTS];    // its byte offset
-    unsigned long long woff[POOL_SLOTS];   // byte offset of the frame to write out
-    int it[POOL_SLOTS];                    // iterations it has run
-    int io[POOL_SLOTS];                    // bit 0: write out, bit 1: load
-    int active;
+    long long *fid;             // frame in the slot, -1 = empty
+    unsigned long long *off;    // its byte offset
+    unsigned long long *woff;   // byte offset of the frame to write out
+    int *it;                    // iterations it has run
+    int *io;                    // bit 0: write out, bit 1: load
+    int *active;                // slots holding a frame
+    uint32_t *keep, *ove, *ovo; // per word: message keep mask, stop-test bits
+    int8_t *tab;                // learned table copy
 };
+
+// Dynamic LDS of a frame-queue workgroup: the posterior block, then the slot
+// bookkeeping and the learned table, sized for this launch (S slots, P words)
+// so that small blocks leave room for one more workgroup per CU.
+__host__ __device__ inline size_t pool_post_bytes(const PoolArgs &a) {
+    return ((size_t)a.N * a.P * (a.pairs ? 2 : 4) + 7) & ~(size_t)7;
+}
+__host__ __device__ inline int pool_slots(const PoolArgs &a) { return a.P * (a.pairs ? 2 : 4); }
+size_t pool_lds_bytes(const PoolArgs &a) {
+    const size_t S = (size_t)pool_slots(a);
+    return pool_post_bytes(a) + S * 24 + S * 8 + 8 + (size_t)a.P * 12 + (a.ltable ? (size_t)a.tits * a.tdeg : 0);
+}
+__device__ __forceinline__ SlotState carve_slots(const PoolArgs &a, char *lds) {
+    const int S = pool_slots(a);
+    char *q = lds + pool_post_bytes(a);
+    SlotState st;
+    st.fid = reinterpret_cast<long long *>(q);
+    q += 8 * S;
+    st.off = reinterpret_cast<unsigned long long *>(q);
+    q += 8 * S;
+    st.woff = reinterpret_cast<unsigned long long *>(q);
+    q += 8 * S;
+    st.it = reinterpret_cast<int *>(q);
+    q += 4 * S;
+    st.io = reinterpret_cast<int *>(q);
+    q += 4 * S;
+    st.active = reinterpret_cast<int *>(q);
+    q += 8;
+    st.keep = reinterpret_cast<uint32_t *>(q);
+    q += 4 * a.P;
+    st.ove = reinterpret_cast<uint32_t *>(q);
+    q += 4 * a.P;
+    st.ovo = reinterpret_cast<uint32_t *>(q);
+    q += 4 * a.P;
+    st.tab = reinterpret_cast<int8_t *>(q);
+    return st;
+}
 
 // thread 0: stop decisions for resident frames (odd_bit(s) = its stop test
 // failed), iteration counts, refill from the queue; returns nothing, fills st
 template <typename OddBit>
-__device__ __forceinline__ void pool_round(const PoolArgs &a, SlotState &st, int S, OddBit odd_bit) {
+__device__ __forceinline__ void pool_round(const PoolArgs &a, const SlotState &st, int S, OddBit odd_bit) {
     int n_free = 0;
     for (int s = 0; s < S; s++) {
         st.io[s] = 0;
@@ -155,7 +219,7 @@ __device__ __forceinline__ void pool_round(const PoolArgs &a, SlotState &st, int
         }
         active += st.fid[s] >= 0;
     }
-    st.active = active;
+    *st.active = active;
 }
 
 // every thread: decisions of the frames leaving their slots, LLRs of the
@@ -200,10 +264,9 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
 template <bool EQ, int DMAXC, int UD, int G>
 __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_pool(const PoolArgs a) {
     extern __shared__ uint32_t lds_post[];
-    __shared__ SlotState st;
-    __shared__ uint32_t s_keep[MAX_P];
-    __shared__ uint32_t s_ove[MAX_P], s_ovo[MAX_P];
-    __shared__ int8_t s_tab[POOL_TABLE];
+    const SlotState st = carve_slots(a, reinterpret_cast<char *>(lds_post));
+    uint32_t *const s_keep = st.keep, *const s_ove = st.ove, *const s_ovo = st.ovo;
+    int8_t *const s_tab = st.tab;
 
     const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P;
     const int S = 4 * P;
@@ -243,7 +306,7 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
         // ---- decisions out, LLRs in
         pool_io<4>(a, st, pb, S, P);
         __syncthreads();
-        if (st.active == 0) break;
+        if (*st.active == 0) break;
 
         // ---- one layered iteration over all slots
         const uint32_t keep = s_keep[w];
@@ -295,11 +358,26 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
             }
         }
         if (a.stop_rule == 0) {  // syndrome of the decisions after the iteration
-            ove = syndrome_items<UD>(a, post, tid >> lp, cstride, lp, w);
-            ovo = 0;
+            const uint2 sy = syndrome_items<UD>(a, post, lp, w);
+            if (UD > 0 && P == 2) {  // per check: both words; one atomic per wave and word
+                uint32_t r0 = 0, r1 = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t bit = 1u << (8 * k + 7);
+                    if (__ballot((sy.x & bit) != 0)) r0 |= bit;
+                    if (__ballot((sy.y & bit) != 0)) r1 |= bit;
+                }
+                if ((tid & 63) == 0) {
+                    if (r0) atomicOr(&s_ove[0], r0);
+                    if (r1) atomicOr(&s_ove[1], r1);
+                }
+            } else if (sy.x) {
+                atomicOr(&s_ove[w], sy.x);
+            }
+        } else {
+            if (ove) atomicOr(&s_ove[w], ove);
+            if (ovo) atomicOr(&s_ovo[w], ovo);
         }
-        if (ove) atomicOr(&s_ove[w], ove);
-        if (ovo) atomicOr(&s_ovo[w], ovo);
         __syncthreads();
     }
 }
@@ -364,10 +442,10 @@ __device__ __forceinline__ uint32_t pair_check_any(int D, uint16_t *post, const 
 template <bool EQ, int DMAXC>
 __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const PoolArgs a) {
     extern __shared__ uint16_t lds_post16[];
-    __shared__ SlotState st;
-    __shared__ uint32_t s_keep[MAX_P];  // per pair: 0xFF per byte whose messages are kept
-    __shared__ uint32_t s_ov[MAX_P];
-    __shared__ int8_t s_tab[POOL_TABLE];
+    const SlotState st = carve_slots(a, reinterpret_cast<char *>(lds_post16));
+    uint32_t *const s_keep = st.keep;  // per pair: 0xFF per byte whose messages are kept
+    uint32_t *const s_ov = st.ove;
+    int8_t *const s_tab = st.tab;
 
     const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P;
     const int S = 2 * P;
@@ -405,7 +483,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
         // ---- decisions out, LLRs in
         pool_io<2>(a, st, pb, S, P);
         __syncthreads();
-        if (st.active == 0) break;
+        if (*st.active == 0) break;
 
         // ---- one layered iteration over all slots
         const u16x2 keep = pair_to_half(s_keep[w]);
@@ -437,8 +515,8 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                 const int D = (int)(ci.x & 0xFFu);
                 const uint32_t *ev = a.edge_var + ci.y;
                 uint32_t par = 0;
-                for (int j = 0; j < D; j++) par ^= positive_bytes(post[(ev[j] << lp) + (uint32_t)w]);
-                ov |= par;
+                for (int j = 0; j < D; j++) par ^= positive_msb(post[(ev[j] << lp) + (uint32_t)w]);
+                ov |= par & 0x8080u;
             }
         }
         if (ov) atomicOr(&s_ov[w], ov);
@@ -448,7 +526,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
 
 template <bool EQ, int DMAXC>
 static hipError_t launch_pairs_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
-    hipLaunchKernelGGL((k_pool_pairs<EQ, DMAXC>), dim3(grid), dim3(threads), (size_t)a.N * a.P * 2, s, a);
+    hipLaunchKernelGGL((k_pool_pairs<EQ, DMAXC>), dim3(grid), dim3(threads), pool_lds_bytes(a), s, a);
     return hipGetLastError();
 }
 
@@ -461,7 +539,7 @@ static const void *pairs_kernel(int dmax) {
 
 template <bool EQ, int DMAXC, int UD, int G>
 static hipError_t launch_pool_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
-    hipLaunchKernelGGL((k_pool<EQ, DMAXC, UD, G>), dim3(grid), dim3(threads), (size_t)a.N * a.P * 4, s, a);
+    hipLaunchKernelGGL((k_pool<EQ, DMAXC, UD, G>), dim3(grid), dim3(threads), pool_lds_bytes(a), s, a);
     return hipGetLastError();
 }
 
@@ -514,8 +592,8 @@ int pool_prepare(const PoolArgs &a, int dmax, int threads) {
     const bool eq = a.variant == VAR_2NMS;
     const void *f = a.pairs ? (eq ? pairs_kernel<true>(dmax) : pairs_kernel<false>(dmax))
                             : (eq ? pool_kernel<true>(dmax, ud, a.batch) : pool_kernel<false>(dmax, ud, a.batch));
-    const size_t lds = (size_t)a.N * a.P * (a.pairs ? 2 : 4);
-    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)POOL_LDS_DYNAMIC) != hipSuccess)
+    const size_t lds = pool_lds_bytes(a);
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MAX_LDS_BYTES) != hipSuccess)
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, lds) != hipSuccess) return 0;
     return n;
