@@ -201,3 +201,24 @@ def test_pool_ira_n64800_auto_pairs(gpu_available):
     assert cfg["pairs"] == 1 and cfg["words_per_block"] == 1
     its = check_pool(c, L.OMS_V2, llr, 50)
     assert its.max() < 50
+
+
+# ---------------------------------------------------------------------------
+# size limits: the largest frames each posterior format holds in LDS
+# (posterior block N*4*P bytes for words, N*2*P for pairs, at most 152 KiB)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,pairs", [(38910, 0), (77820, 1)], ids=["words_N38910", "pairs_N77820"])
+def test_pool_largest_frames(n, pairs, gpu_available):
+    c = L.code_3_6(n, seed=9)
+    pool = L.PoolDecoder(c, L.OMS_V2, 12, pairs=pairs, words_per_block=1)
+    cfg = pool.config()
+    assert cfg["pairs"] == pairs and cfg["words_per_block"] == 1
+    llr = mixed_llr(n, 24, seed=3, lo=1.8, hi=3.0)
+    check_pool(c, L.OMS_V2, llr, 12, pairs=pairs, words_per_block=1)
+
+
+def test_pool_rejects_blocks_beyond_lds(gpu_available):
+    c = L.code_3_6(38916, seed=9)  # N*4 just above the posterior budget
+    with pytest.raises(L.LdpcError):
+        L.PoolDecoder(c, L.OMS_V2, 10, pairs=0)
+    assert L.PoolDecoder(c, L.OMS_V2, 10).config()["pairs"] == 1  # auto: pair mode
