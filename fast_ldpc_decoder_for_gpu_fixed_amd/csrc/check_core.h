@@ -200,14 +200,17 @@ __device__ __forceinline__ void store_msgs(uint32_t *__restrict__ mp, const uint
 // (edge_off16), and the index and message loads of G items are issued
 // together before any of them is computed.  FIRST: no message read (the
 // decode's first iteration).  keep: byte mask applied to every message word
-// read (a lane whose frame starts now reads zero messages).  Returns the
-// OR of the items' parity bits (check_core's layout).
+// read (a lane whose frame starts now reads zero messages).  VK >= 0: the
+// variant known at compile time (its magnitude map is then straight-line code
+// with no per-item branch), -1: read from `variant`.  Returns the OR of the
+// items' parity bits (check_core's layout).
 // ---------------------------------------------------------------------------
-template <int D, int G, bool EQ, bool FIRST>
+template <int D, int G, bool EQ, bool FIRST, int VK = -1>
 __device__ __forceinline__ uint2 uniform_items(const uint32_t *__restrict__ edge_off16, int P, int variant,
                                                uint32_t *post, uint32_t *mbase, int c_first, int le, int cstride,
                                                int w, uint32_t keep, u16x2 lpe, u16x2 lpo) {
     constexpr int H = (D + 1) / 2;
+    if constexpr (VK >= 0) variant = VK;
     char *const pbase = reinterpret_cast<char *>(post) + 4 * w;
     uint2 ov = make_uint2(0u, 0u);
     for (int c0 = c_first; c0 < le; c0 += G * cstride) {

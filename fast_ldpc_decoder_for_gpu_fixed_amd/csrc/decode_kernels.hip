@@ -50,11 +50,11 @@ __device__ __forceinline__ uint32_t check_d(PostPtr post, const uint32_t *__rest
 
 // One level's items of one thread for a code whose checks all have degree D
 // (the shared body is uniform_items in check_core.h).
-template <int D, int G, bool EQ, bool FIRST>
+template <int D, int G, bool EQ, bool FIRST, int VK>
 __device__ __forceinline__ uint32_t level_uniform(const DecodeArgs &a, uint32_t *post, uint32_t *mbase,
                                                   int c_first, int le, int cstride, int w, int lparam) {
     const u16x2 lpv = splat(learned_half(a.variant, lparam));
-    const uint2 pr = uniform_items<D, G, EQ, FIRST>(a.edge_off16, a.P, a.variant, post, mbase, c_first, le, cstride,
+    const uint2 pr = uniform_items<D, G, EQ, FIRST, VK>(a.edge_off16, a.P, a.variant, post, mbase, c_first, le, cstride,
                                                     w, 0xFFFFFFFFu, lpv, lpv);
     return pr.x | pr.y;
 }
@@ -160,7 +160,7 @@ __device__ void store_block(const DecodeArgs &a, const uint32_t *post, size_t b)
 // ---------------------------------------------------------------------------
 // the decode kernel: one workgroup per block of P words, all iterations
 // ---------------------------------------------------------------------------
-template <bool EQ, bool POST_IN_LDS, int DMAXC, int UD, int G>
+template <bool EQ, bool POST_IN_LDS, int DMAXC, int UD, int G, int VK>
 __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_decode(const DecodeArgs a) {
     extern __shared__ uint32_t lds_post[];
     __shared__ uint32_t s_ov[MAX_P];
@@ -217,9 +217,9 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_de
                 if (act) {
                     const int lpar = a.ltable ? (int)s_lrow[UD] : 0;
                     if (first)
-                        ov |= level_uniform<UD, G, EQ, true>(a, post, mbase, lb + (tid >> lp), le, cstride, w, lpar);
+                        ov |= level_uniform<UD, G, EQ, true, VK>(a, post, mbase, lb + (tid >> lp), le, cstride, w, lpar);
                     else
-                        ov |= level_uniform<UD, G, EQ, false>(a, post, mbase, lb + (tid >> lp), le, cstride, w, lpar);
+                        ov |= level_uniform<UD, G, EQ, false, VK>(a, post, mbase, lb + (tid >> lp), le, cstride, w, lpar);
                 }
             } else if (act) {
                 for (int c = lb + (tid >> lp); c < le; c += cstride) {
@@ -314,19 +314,19 @@ __global__ __launch_bounds__(256) void k_deinterleave_hd(const uint32_t *__restr
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <bool EQ, bool LDS, int DMAXC, int UD, int G>
+template <bool EQ, bool LDS, int DMAXC, int UD, int G, int VK = -1>
 static hipError_t launch_decode_t(const DecodeArgs &a, int threads, hipStream_t s) {
     const size_t lds = LDS ? (size_t)a.N * a.P * 4 : 0;
     static std::once_flag flag;
     static hipError_t attr = hipSuccess;
     const hipError_t e = once_per_process(flag, attr, [] {
-        return hipFuncSetAttribute((const void *)k_decode<EQ, LDS, DMAXC, UD, G>,
+        return hipFuncSetAttribute((const void *)k_decode<EQ, LDS, DMAXC, UD, G, VK>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(MAX_LDS_BYTES - 1024));  // static flags live in the rest
     });
     if (e != hipSuccess) return e;
     const unsigned nblk = (unsigned)(a.W / a.P);
-    hipLaunchKernelGGL((k_decode<EQ, LDS, DMAXC, UD, G>), dim3(nblk), dim3(threads), lds, s, a);
+    hipLaunchKernelGGL((k_decode<EQ, LDS, DMAXC, UD, G, VK>), dim3(nblk), dim3(threads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -339,9 +339,24 @@ static hipError_t launch_dclass(const DecodeArgs &a, int dmax, int threads, hipS
 
 template <bool EQ, int UD>
 static hipError_t launch_uniform_g(const DecodeArgs &a, int batch, int threads, hipStream_t s) {
-    if (batch >= 4) return launch_decode_t<EQ, true, 8, UD, 4>(a, threads, s);
-    if (batch == 2) return launch_decode_t<EQ, true, 8, UD, 2>(a, threads, s);
-    return launch_decode_t<EQ, true, 8, UD, 1>(a, threads, s);
+    // the plain OMS family and learned offsets get their own straight-line
+    // instantiations; the other variants read the variant at run time
+    const int vk = (a.variant == VAR_OMS || a.variant == VAR_OMS_V2) ? VAR_OMS : a.variant == VAR_LOMS ? VAR_LOMS : -1;
+#define LDPC_UG(VKC)                                                                              \
+    if (vk == VKC) {                                                                              \
+        if (batch >= 4) return launch_decode_t<EQ, true, 8, UD, 4, VKC>(a, threads, s);           \
+        if (batch == 2) return launch_decode_t<EQ, true, 8, UD, 2, VKC>(a, threads, s);           \
+        return launch_decode_t<EQ, true, 8, UD, 1, VKC>(a, threads, s);                           \
+    }
+    if constexpr (!EQ) {
+        LDPC_UG(VAR_OMS)
+        LDPC_UG(VAR_LOMS)
+    }
+    LDPC_UG(-1)
+#undef LDPC_UG
+    if (batch >= 4) return launch_decode_t<EQ, true, 8, UD, 4, -1>(a, threads, s);
+    if (batch == 2) return launch_decode_t<EQ, true, 8, UD, 2, -1>(a, threads, s);
+    return launch_decode_t<EQ, true, 8, UD, 1, -1>(a, threads, s);
 }
 
 template <bool EQ>

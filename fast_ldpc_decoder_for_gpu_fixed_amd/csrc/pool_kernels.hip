@@ -261,7 +261,7 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
     }
 }
 
-template <bool EQ, int DMAXC, int UD, int G>
+template <bool EQ, int DMAXC, int UD, int G, int VK>
 __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_pool(const PoolArgs a) {
     extern __shared__ uint32_t lds_post[];
     const SlotState st = carve_slots(a, reinterpret_cast<char *>(lds_post));
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
                          (learned_half(a.variant, s_tab[rowoff[3] + col]) << 16));
             }
             for (int L = 0; L < a.n_levels; L++) {
-                const uint2 pr = uniform_items<UD, G, EQ, false>(a.edge_off16, P, a.variant, post, mbase,
+                const uint2 pr = uniform_items<UD, G, EQ, false, VK>(a.edge_off16, P, a.variant, post, mbase,
                                                                  a.level_beg[L] + (tid >> lp), a.level_beg[L + 1],
                                                                  cstride, w, keep, lpe, lpo);
                 ove |= pr.x;
@@ -524,12 +524,6 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
     }
 }
 
-template <bool EQ, int DMAXC>
-static hipError_t launch_pairs_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
-    hipLaunchKernelGGL((k_pool_pairs<EQ, DMAXC>), dim3(grid), dim3(threads), pool_lds_bytes(a), s, a);
-    return hipGetLastError();
-}
-
 template <bool EQ>
 static const void *pairs_kernel(int dmax) {
     if (dmax <= 8) return (const void *)k_pool_pairs<EQ, 8>;
@@ -537,65 +531,52 @@ static const void *pairs_kernel(int dmax) {
     return (const void *)k_pool_pairs<EQ, 32>;
 }
 
-template <bool EQ, int DMAXC, int UD, int G>
-static hipError_t launch_pool_t(const PoolArgs &a, int grid, int threads, hipStream_t s) {
-    hipLaunchKernelGGL((k_pool<EQ, DMAXC, UD, G>), dim3(grid), dim3(threads), pool_lds_bytes(a), s, a);
-    return hipGetLastError();
+template <bool EQ, int VK>
+static const void *uniform_kernel(int ud, int G) {
+#define LDPC_UK(U)                                                                                    \
+    if (ud == U) return G >= 4 ? (const void *)k_pool<EQ, 8, U, 4, VK>                               \
+                               : G == 2 ? (const void *)k_pool<EQ, 8, U, 2, VK> : (const void *)k_pool<EQ, 8, U, 1, VK>;
+    LDPC_UK(6) LDPC_UK(7) LDPC_UK(8)
+#undef LDPC_UK
+    return nullptr;
 }
 
-// the instantiation that runs a pool: generic degree classes, or the
-// single-degree path (ud 6..8, G items in flight per thread)
-template <bool EQ>
-static const void *pool_kernel(int dmax, int ud, int G) {
-    if (ud == 6) return G >= 4 ? (const void *)k_pool<EQ, 8, 6, 4> : G == 2 ? (const void *)k_pool<EQ, 8, 6, 2> : (const void *)k_pool<EQ, 8, 6, 1>;
-    if (ud == 7) return G >= 4 ? (const void *)k_pool<EQ, 8, 7, 4> : G == 2 ? (const void *)k_pool<EQ, 8, 7, 2> : (const void *)k_pool<EQ, 8, 7, 1>;
-    if (ud == 8) return G >= 4 ? (const void *)k_pool<EQ, 8, 8, 4> : G == 2 ? (const void *)k_pool<EQ, 8, 8, 2> : (const void *)k_pool<EQ, 8, 8, 1>;
-    if (dmax <= 8) return (const void *)k_pool<EQ, 8, 0, 1>;
-    if (dmax <= 16) return (const void *)k_pool<EQ, 16, 0, 1>;
-    return (const void *)k_pool<EQ, 32, 0, 1>;
-}
-
-template <bool EQ>
-static hipError_t launch_pool_eq(const PoolArgs &a, int dmax, int ud, int G, int grid, int threads, hipStream_t s) {
-#define LDPC_PL(U, GG)                                                                       \
-    if (ud == U && (GG == 4 ? G >= 4 : G == GG)) return launch_pool_t<EQ, 8, U, GG>(a, grid, threads, s);
-    LDPC_PL(6, 1) LDPC_PL(6, 2) LDPC_PL(6, 4) LDPC_PL(7, 1) LDPC_PL(7, 2) LDPC_PL(7, 4)
-    LDPC_PL(8, 1) LDPC_PL(8, 2) LDPC_PL(8, 4)
-#undef LDPC_PL
-    if (ud > 0) return hipErrorInvalidValue;
-    if (dmax <= 8) return launch_pool_t<EQ, 8, 0, 1>(a, grid, threads, s);
-    if (dmax <= 16) return launch_pool_t<EQ, 16, 0, 1>(a, grid, threads, s);
-    return launch_pool_t<EQ, 32, 0, 1>(a, grid, threads, s);
+// The instantiation that runs a pool: pair mode by degree class; the
+// single-degree path (degree 6..8, G items in flight per thread), with the
+// plain OMS family and learned offsets compiled straight-line; or the generic
+// degree classes.
+static const void *pool_kernel(const PoolArgs &a, int dmax) {
+    const bool eq = a.variant == VAR_2NMS;
+    if (a.pairs) return eq ? pairs_kernel<true>(dmax) : pairs_kernel<false>(dmax);
+    if (a.uniform_deg) {
+        if (!uniform_degree_supported(a.uniform_deg) || a.uniform_deg != dmax) return nullptr;
+        if (eq) return uniform_kernel<true, -1>(a.uniform_deg, a.batch);
+        if (a.variant == VAR_OMS || a.variant == VAR_OMS_V2) return uniform_kernel<false, VAR_OMS>(a.uniform_deg, a.batch);
+        if (a.variant == VAR_LOMS) return uniform_kernel<false, VAR_LOMS>(a.uniform_deg, a.batch);
+        return uniform_kernel<false, -1>(a.uniform_deg, a.batch);
+    }
+    if (dmax <= 8) return eq ? (const void *)k_pool<true, 8, 0, 1, -1> : (const void *)k_pool<false, 8, 0, 1, -1>;
+    if (dmax <= 16) return eq ? (const void *)k_pool<true, 16, 0, 1, -1> : (const void *)k_pool<false, 16, 0, 1, -1>;
+    return eq ? (const void *)k_pool<true, 32, 0, 1, -1> : (const void *)k_pool<false, 32, 0, 1, -1>;
 }
 
 hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s) {
-    if (a.pairs) {
-        if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax, false) || grid < 1 || a.uniform_deg)
-            return hipErrorInvalidValue;
-        const bool eq = a.variant == VAR_2NMS;
-        if (dmax <= 8) return eq ? launch_pairs_t<true, 8>(a, grid, threads, s) : launch_pairs_t<false, 8>(a, grid, threads, s);
-        if (dmax <= 16) return eq ? launch_pairs_t<true, 16>(a, grid, threads, s) : launch_pairs_t<false, 16>(a, grid, threads, s);
-        return eq ? launch_pairs_t<true, 32>(a, grid, threads, s) : launch_pairs_t<false, 32>(a, grid, threads, s);
-    }
-    const int ud = a.uniform_deg;
-    if (ud > 0 && !a.edge_off16) return hipErrorInvalidValue;
-    if (dmax < 1 || dmax > MAX_DEG || threads > max_threads_for_degree(dmax, ud > 0) || grid < 1)
+    if (dmax < 1 || dmax > MAX_DEG || grid < 1 || threads > max_threads_for_degree(dmax, a.uniform_deg > 0))
         return hipErrorInvalidValue;
-    if (ud > 0 && (ud != dmax || !uniform_degree_supported(ud))) return hipErrorInvalidValue;
-    return a.variant == VAR_2NMS ? launch_pool_eq<true>(a, dmax, ud, a.batch, grid, threads, s)
-                                 : launch_pool_eq<false>(a, dmax, ud, a.batch, grid, threads, s);
+    if (a.uniform_deg && !a.edge_off16) return hipErrorInvalidValue;  // the path reads its offset table
+    const void *f = pool_kernel(a, dmax);
+    if (!f) return hipErrorInvalidValue;
+    void *args[] = {const_cast<PoolArgs *>(&a)};
+    return hipLaunchKernel(f, dim3(grid), dim3(threads), args, pool_lds_bytes(a), s);
 }
 
 int pool_prepare(const PoolArgs &a, int dmax, int threads) {
     int n = 0;
-    const int ud = a.uniform_deg;
-    const bool eq = a.variant == VAR_2NMS;
-    const void *f = a.pairs ? (eq ? pairs_kernel<true>(dmax) : pairs_kernel<false>(dmax))
-                            : (eq ? pool_kernel<true>(dmax, ud, a.batch) : pool_kernel<false>(dmax, ud, a.batch));
-    const size_t lds = pool_lds_bytes(a);
+    const void *f = pool_kernel(a, dmax);
+    if (!f) return 0;
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MAX_LDS_BYTES) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, lds) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, pool_lds_bytes(a)) != hipSuccess) return 0;
     return n;
 }
 
