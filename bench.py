@@ -95,10 +95,19 @@ def cpu_baseline(code, llr, n_words, iters):
     orc.iterate(iters, threads=threads)
     dt = time.perf_counter() - t0
     frames = 4 * n_words
+    # single-core figure on the first 1024 words (4096 frames)
+    w1 = min(1024, n_words)
+    one = O.OracleDecoder(deg, pos, N, w1, O.OMS, q1_split=code.q1_split, early_term=0)
+    one.load(llr[: 4 * w1 * N])
+    t0 = time.perf_counter()
+    one.iterate(iters, threads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": round(frames * code.n_info / dt / 1e9, 4), "unit": "info Gbit/s", "cores": threads,
             "kind": "port",
             "sample": f"{frames} frames ({n_words} words) of the same code and input, plain OMS, {iters} fixed "
-                      f"iterations, {dt:.2f} s on {threads} host threads"}
+                      f"iterations, {dt:.2f} s on {threads} host threads",
+            "single_core": {"value": round(4 * w1 * code.n_info / dt1 / 1e9, 5), "unit": "info Gbit/s",
+                            "sample": f"first {4 * w1} frames, {dt1:.2f} s on 1 thread"}}
 
 
 def timed(L, stream, steps, warmup, fn):
@@ -228,6 +237,15 @@ def run_plain(L, args, t_start):
     ms = timed(L, dec.stream, args.steps, args.warmup, lambda: dec.decode_device(d_in.ptr, d_out.ptr, args.iters))
     info_gbps = frames * K / (ms * 1e-3) / 1e9
     alg = frames * (args.iters * (2 * E + 2 * N) + 2 * N)
+    # the drop-in's host-buffer decode (pinned in/out): PCIe copies included, never `value`
+    pin_in, pin_out = L.PinnedBuffer(frames * N), L.PinnedBuffer(frames * N)
+    d_in.download(pin_in.array)
+    dec.decode(pin_in.array, pin_out.array, args.iters)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dec.decode(pin_in.array, pin_out.array, args.iters)
+    ms_pcie = (time.perf_counter() - t0) * 1e3 / 3
+    same = bool(np.array_equal(pin_out.array, d_out.download(np.empty(frames * N, np.uint8))))
     out = {
         "metric": "info Gbit/s (plain %s, (3,6) N=4000, %d it, %d frames, 1 MI355X); HBM GB/s"
                   % (L.VARIANT_NAMES[variant], args.iters, frames),
@@ -248,7 +266,11 @@ def run_plain(L, args, t_start):
                    "frames": frames, "iterations": args.iters, "variant": L.VARIANT_NAMES[variant],
                    "words_per_block": cfg["words_per_block"], "threads": cfg["threads"],
                    "single_degree_kernel": cfg["uniform_deg"] > 0, "batch": cfg["batch"],
-                   "levels_per_iteration": cfg["n_levels"], "parallelism": "single GPU"},
+                   "levels_per_iteration": cfg["n_levels"], "parallelism": "single GPU",
+                   "pcie_inclusive": {"ms_per_decode": round(ms_pcie, 3),
+                                      "info_gbps": round(frames * K / (ms_pcie * 1e-3) / 1e9, 3),
+                                      "buffers": "pinned host in/out, H2D + decode + D2H, synchronous",
+                                      "decisions_equal_device_path": same}},
         "roofline": roofline(alg, ms, traffic_for(args.traffic_json, f"oms_3_6_n4000_f{frames}_it{args.iters}")
                              if variant == L.OMS else None),
     }

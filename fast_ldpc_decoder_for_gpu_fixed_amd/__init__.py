@@ -339,11 +339,32 @@ class DeviceBuffer:
             self.ptr = None
 
 
+class PinnedBuffer:
+    """Page-locked host memory (hipHostMalloc), the counterpart of the
+    reference's CTrame buffers (src/trame/CTrame.cpp:38-41).  `array` is a
+    uint8 view of it and must not be used after the buffer is dropped."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = nbytes
+        self.ptr = load().ldpc_host_alloc_pinned(nbytes)
+        if not self.ptr:
+            raise LdpcError(f"pinned alloc of {nbytes} bytes failed: {_lib.last_error()}")
+        self.array = np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(self.ptr))
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p and _lib._lib is not None:
+            self.array = None
+            _lib._lib.ldpc_host_free_pinned(C.c_void_p(p))
+            self.ptr = None
+
+
 class PoolDecoder:
     """Persistent frame-queue decoder: each workgroup keeps 4P frames
-    resident, stops every frame on its own (every check with an even parity
-    of positive v2c values, or max_iterations) and refills its slot from a
-    device queue.  Inputs and outputs stay on the device (frame-major)."""
+    resident, stops every frame on its own (by default when its decisions
+    satisfy every check; STOP_V2C_PARITY selects the reference's 2NMS rule;
+    else at max_iterations) and refills its slot from a device queue.  Inputs
+    and outputs stay on the device (frame-major)."""
 
     def __init__(self, code: Code, variant: int, max_iterations: int, words_per_block: int = 0,
                  threads: int = 0, table=None, uniform: int = -1, batch: int = 0, stop_rule: int = 0,

@@ -185,49 +185,79 @@ __device__ __forceinline__ SlotState carve_slots(const PoolArgs &a, char *lds) {
     return st;
 }
 
-// thread 0: stop decisions for resident frames (odd_bit(s) = its stop test
-// failed), iteration counts, refill from the queue; returns nothing, fills st
-template <typename OddBit>
-__device__ __forceinline__ void pool_round(const PoolArgs &a, const SlotState &st, int S, OddBit odd_bit) {
-    int n_free = 0;
-    for (int s = 0; s < S; s++) {
-        st.io[s] = 0;
-        if (st.fid[s] >= 0) {
-            const int it = ++st.it[s];
-            if (!odd_bit(s) || it >= a.max_iters) {
-                if (a.out_iters) a.out_iters[st.fid[s]] = it;
-                st.io[s] = 1;
-                st.woff[s] = st.off[s];
-                st.fid[s] = -1;
-            }
+// wave 0, lane s = slot s (S <= 64): stop decisions for resident frames
+// (odd_bit(s) = its stop test failed), iteration counts, and the refill from
+// the queue in slot order with one atomic per round; then the per-word keep
+// masks (B byte lanes per word: a refilled lane's messages are cleared) and
+// the reset of the per-word stop-test bits (`reset(q)`)
+template <int B, typename OddBit, typename Reset>
+__device__ __forceinline__ void pool_round(const PoolArgs &a, const SlotState &st, int S, int P, OddBit odd_bit,
+                                           Reset reset) {
+    // opaque lane index: keeps per-lane slot addresses from being hoisted
+    // out of the decoder loop (they would stay live in VGPRs through it)
+    int s = threadIdx.x;
+    __asm__ volatile("" : "+v"(s));
+    const bool mine = s < S;
+    long long fid = mine ? st.fid[s] : -1;
+    unsigned long long off = mine ? st.off[s] : 0ull;
+    int it = mine ? st.it[s] : 0, io = 0;
+    if (fid >= 0) {
+        it++;
+        if (!odd_bit(s) || it >= a.max_iters) {
+            if (a.out_iters) a.out_iters[fid] = it;
+            io = 1;
+            st.woff[s] = off;
+            fid = -1;
         }
-        n_free += st.fid[s] < 0;
     }
+    const unsigned long long free_mask = __ballot(mine && fid < 0);
+    const int n_free = __popcll(free_mask);
     const unsigned long long total = a.total_ptr ? (unsigned long long)*a.total_ptr : a.total;
     // one atomic per round (a read of the head before it would add a second
     // round trip to this serial path; the head may pass `total`, harmlessly)
-    unsigned long long base = n_free ? atomicAdd(a.next, (unsigned long long)n_free) : total;
-    int active = 0;
-    for (int s = 0; s < S; s++) {
-        if (st.fid[s] < 0 && base < total) {
-            const unsigned long long f = a.frame_list ? (unsigned long long)a.frame_list[base] : base;
-            st.fid[s] = (long long)f;
-            st.off[s] = a.frame_off ? a.frame_off[f] : f * (unsigned long long)a.N;
-            st.it[s] = 0;
-            st.io[s] |= 2;
-            base++;
+    unsigned long long base = total;
+    if (s == 0 && n_free) base = atomicAdd(a.next, (unsigned long long)n_free);
+    base = __shfl(base, 0);
+    if (mine && fid < 0) {
+        const unsigned long long q = base + (unsigned long long)__popcll(free_mask & ((1ull << s) - 1ull));
+        if (q < total) {
+            const unsigned long long f = a.frame_list ? (unsigned long long)a.frame_list[q] : q;
+            fid = (long long)f;
+            off = a.frame_off ? a.frame_off[f] : f * (unsigned long long)a.N;
+            it = 0;
+            io |= 2;
         }
-        active += st.fid[s] >= 0;
     }
-    *st.active = active;
+    if (mine) {
+        st.fid[s] = fid;
+        st.off[s] = off;
+        st.it[s] = it;
+        st.io[s] = io;
+    }
+    const unsigned long long active = __ballot(mine && fid >= 0);
+    const unsigned long long fresh = __ballot((io & 2) != 0);
+    if (s == 0) *st.active = __popcll(active);
+    if (s < P) {
+        uint32_t keep = B == 4 ? 0xFFFFFFFFu : 0xFFFFu;
+#pragma unroll
+        for (int k = 0; k < B; k++)
+            if ((fresh >> (B * s + k)) & 1ull) keep &= ~(0xFFu << (8 * k));
+        st.keep[s] = keep;
+        reset(s);
+    }
 }
 
 // every thread: decisions of the frames leaving their slots, LLRs of the
 // frames entering them (byte lane k of word wl of each variable; B = bytes
-// per lane group: 4 for words, 2 for pairs)
+// per lane group: 4 for words, 2 for pairs).  The LLR loads of one slot are
+// issued IO_BATCH dwords per thread at a time before any is stored.
+constexpr int IO_BATCH = 8;
 template <int B>
 __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, uint8_t *pb, int S, int P) {
-    const int tid = threadIdx.x, T = blockDim.x, N = a.N;
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));  // as in pool_round: no per-thread addresses hoisted
+    const int T = blockDim.x, N = a.N;
+    const size_t stride = (size_t)P * B;
     for (int s = 0; s < S; s++) {
         const int io = st.io[s];
         if (!io) continue;  // uniform: read after a barrier
@@ -236,24 +266,36 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
         if ((N & 3) == 0 && ((wo | lo) & 3) == 0) {
             uint32_t *dst = reinterpret_cast<uint32_t *>(a.out_hard + wo);
             const uint32_t *src = reinterpret_cast<const uint32_t *>(a.in_frames + lo);
-            for (int q = tid; q < (N >> 2); q += T) {
-                uint8_t *lane = pb + ((size_t)(4 * q) * P + wl) * B + k;
-                const size_t stride = (size_t)P * B;
-                if (io & 1) {
-                    uint32_t x = 0;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) x |= (uint32_t)((int8_t)lane[r * stride] > 0) << (8 * r);
-                    dst[q] = x;
-                }
+            const int Q = N >> 2;
+            for (int q0 = tid; q0 < Q; q0 += IO_BATCH * T) {
+                uint32_t y[IO_BATCH];
                 if (io & 2) {
-                    const uint32_t y = src[q];
 #pragma unroll
-                    for (int r = 0; r < 4; r++) lane[r * stride] = (uint8_t)(y >> (8 * r));
+                    for (int r = 0; r < IO_BATCH; r++) {
+                        const int q = q0 + r * T;
+                        y[r] = q < Q ? src[q] : 0u;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < IO_BATCH; r++) {
+                    int q = q0 + r * T;
+                    __asm__ volatile("" : "+v"(q));  // addresses formed here, one at a time
+                    if (q >= Q) break;
+                    uint8_t *lane = pb + (size_t)(4 * q) * stride + (size_t)wl * B + k;
+                    if (io & 1) {
+                        uint32_t x = 0;
+#pragma unroll
+                        for (int b = 0; b < 4; b++) x |= (uint32_t)((int8_t)lane[b * stride] > 0) << (8 * b);
+                        dst[q] = x;
+                    }
+                    if (io & 2)
+#pragma unroll
+                        for (int b = 0; b < 4; b++) lane[b * stride] = (uint8_t)(y[r] >> (8 * b));
                 }
             }
         } else {
             for (int v = tid; v < N; v += T) {
-                uint8_t *lane = pb + ((size_t)v * P + wl) * B + k;
+                uint8_t *lane = pb + (size_t)v * stride + (size_t)wl * B + k;
                 if (io & 1) a.out_hard[wo + v] = (uint8_t)((int8_t)*lane > 0);
                 if (io & 2) *lane = (uint8_t)a.in_frames[lo + v];
             }
@@ -286,22 +328,17 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
     __syncthreads();
 
     for (;;) {
-        // ---- thread 0: stop decisions, iteration counts, refill
-        if (tid == 0) {
-            pool_round(a, st, S, [&](int s) {
-                const int wl = s >> 2, k = s & 3;
-                if (a.stop_rule == 0) return (bool)((s_ove[wl] >> (8 * k + 7)) & 1u);
-                const uint32_t src = (k & 1) ? s_ovo[wl] : s_ove[wl];
-                return (bool)((src >> ((k & 2) ? 31 : 15)) & 1u);
-            });
-            for (int q = 0; q < P; q++) {
-                uint32_t keep = 0xFFFFFFFFu;
-                for (int k = 0; k < 4; k++)
-                    if (st.io[4 * q + k] & 2) keep &= ~(0xFFu << (8 * k));
-                s_keep[q] = keep;
-                s_ove[q] = s_ovo[q] = 0;
-            }
-        }
+        // ---- wave 0: stop decisions, iteration counts, refill
+        if (tid < 64)
+            pool_round<4>(
+                a, st, S, P,
+                [&](int s) {
+                    const int wl = s >> 2, k = s & 3;
+                    if (a.stop_rule == 0) return (bool)((s_ove[wl] >> (8 * k + 7)) & 1u);
+                    const uint32_t src = (k & 1) ? s_ovo[wl] : s_ove[wl];
+                    return (bool)((src >> ((k & 2) ? 31 : 15)) & 1u);
+                },
+                [&](int q) { s_ove[q] = s_ovo[q] = 0; });
         __syncthreads();
         // ---- decisions out, LLRs in
         pool_io<4>(a, st, pb, S, P);
@@ -465,20 +502,15 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
     __syncthreads();
 
     for (;;) {
-        // ---- thread 0: stop decisions, iteration counts, refill
-        if (tid == 0) {
-            pool_round(a, st, S, [&](int s) {
-                const uint32_t bit = a.stop_rule == 0 ? (8u * (s & 1) + 7u) : ((s & 1) ? 31u : 15u);
-                return (bool)((s_ov[s >> 1] >> bit) & 1u);
-            });
-            for (int q = 0; q < P; q++) {
-                uint32_t keep = 0xFFFFu;
-                for (int k = 0; k < 2; k++)
-                    if (st.io[2 * q + k] & 2) keep &= ~(0xFFu << (8 * k));
-                s_keep[q] = keep;
-                s_ov[q] = 0;
-            }
-        }
+        // ---- wave 0: stop decisions, iteration counts, refill
+        if (tid < 64)
+            pool_round<2>(
+                a, st, S, P,
+                [&](int s) {
+                    const uint32_t bit = a.stop_rule == 0 ? (8u * (s & 1) + 7u) : ((s & 1) ? 31u : 15u);
+                    return (bool)((s_ov[s >> 1] >> bit) & 1u);
+                },
+                [&](int q) { s_ov[q] = 0; });
         __syncthreads();
         // ---- decisions out, LLRs in
         pool_io<2>(a, st, pb, S, P);
