@@ -23,6 +23,10 @@ enum LoadMode { LOAD_FRAMES = 0, LOAD_WORDS = 1, LOAD_STATE = 2 };
 
 constexpr int MAX_P = 16;                  // words per workgroup block
 constexpr int MAX_DEG = 32;                // largest check degree handled
+// The frame-queue kernels gather a check's variable ids and messages for
+// the largest degree of their class without a bound test, so the edge-id and
+// message arrays carry this many spare entries (u32) past their end.
+constexpr int GATHER_PAD = MAX_DEG;
 constexpr size_t MAX_LDS_BYTES = 160 * 1024;
 
 struct DecodeArgs {

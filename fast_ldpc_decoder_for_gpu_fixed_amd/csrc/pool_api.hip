@@ -197,14 +197,15 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
         if (e == hipSuccess && n) e = hipMalloc((void **)q, n * sizeof(**q));
     };
     al(&p->d_checks, ck.size());
-    al(&p->d_edge_var, (size_t)code->E);
+    al(&p->d_edge_var, (size_t)code->E + ldpc::GATHER_PAD);
     al(&p->d_level_beg, p->plan.level_beg.size());
-    al(&p->d_msgs, ((size_t)p->max_grid * code->E * P * wb / code->N + 3) / 4);  // u32 units
+    al(&p->d_msgs, ((size_t)p->max_grid * code->E * P * wb / code->N + 3) / 4 + ldpc::GATHER_PAD);  // u32 units
     al(&p->d_next, 1);
     al(&p->d_edge_off16, off16.size());
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamDefault);
     if (e == hipSuccess) e = hipMemcpy(p->d_checks, ck.data(), ck.size() * sizeof(uint2), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_edge_var, p->plan.edge_var.data(), (size_t)code->E * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(p->d_edge_var + code->E, 0, (size_t)ldpc::GATHER_PAD * 4);
     if (e == hipSuccess)
         e = hipMemcpy(p->d_level_beg, p->plan.level_beg.data(), p->plan.level_beg.size() * sizeof(int),
                       hipMemcpyHostToDevice);

@@ -21,9 +21,6 @@
 
 namespace ldpc {
 
-constexpr int POOL_SLOTS = 4 * MAX_P;
-constexpr int POOL_TABLE = 64 * (MAX_DEG + 1);  // learned table bytes at most (kept in LDS)
-
 template <int D, bool EQ>
 __device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__restrict__ ev, uint32_t *__restrict__ mp,
                                             int lp, int w, uint32_t keep, int variant, u16x2 lpe, u16x2 lpo) {
@@ -62,21 +59,32 @@ __device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_
     if constexpr (UD > 0) {
         constexpr int H = (UD + 1) / 2;
         if (a.P == 2) {
+            // SB checks per thread at a time: their offsets are loaded together
+            constexpr int SB = 4;
             const char *pbase = reinterpret_cast<const char *>(post);
-            for (int c = tid; c < a.C; c += T) {
-                uint32_t off[H];
+            for (int c0 = tid; c0 < a.C; c0 += SB * T) {
+                uint32_t off[SB][H];
 #pragma unroll
-                for (int h = 0; h < H; h++) off[h] = a.edge_off16[(size_t)c * H + h];
-                uint32_t p0 = 0, p1 = 0;
+                for (int b = 0; b < SB; b++) {
+                    const int c = min(c0 + b * T, a.C - 1);
 #pragma unroll
-                for (int j = 0; j < UD; j++) {
-                    const uint32_t o = (j & 1) ? (off[j >> 1] >> 16) : (off[j >> 1] & 0xFFFFu);
-                    const uint2 x = *reinterpret_cast<const uint2 *>(pbase + o);
-                    p0 ^= positive_msb(x.x);
-                    p1 ^= positive_msb(x.y);
+                    for (int h = 0; h < H; h++) off[b][h] = a.edge_off16[(size_t)c * H + h];
                 }
-                synd.x |= p0;
-                synd.y |= p1;
+#pragma unroll
+                for (int b = 0; b < SB; b++) {
+                    uint32_t p0 = 0, p1 = 0;
+#pragma unroll
+                    for (int j = 0; j < UD; j++) {
+                        const uint32_t o = (j & 1) ? (off[b][j >> 1] >> 16) : (off[b][j >> 1] & 0xFFFFu);
+                        const uint2 x = *reinterpret_cast<const uint2 *>(pbase + o);
+                        p0 ^= positive_msb(x.x);
+                        p1 ^= positive_msb(x.y);
+                    }
+                    // a clamped index repeats check C-1: OR-ing its parity twice is harmless
+                    synd.x |= p0;
+                    synd.y |= p1;
+                    __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
+                }
             }
         } else {
             const char *pbase = reinterpret_cast<const char *>(post) + 4 * w;
@@ -130,8 +138,8 @@ __device__ __forceinline__ uint2 pool_check_any(int D, uint32_t *post, const uin
     }
 }
 
-// Slot bookkeeping of one workgroup, kept by thread 0 between barriers.
-// Per round: thread 0 decides which resident frames stop (their stop test or
+// Slot bookkeeping of one workgroup, kept by wave 0 between barriers.
+// Per round: wave 0 decides which resident frames stop (their stop test or
 // the iteration limit), writes their iteration counts, and hands the free
 // slots the next frames of the queue (one atomicAdd); then every thread
 // writes the stopped frames' decisions and loads the new frames' LLRs for
@@ -258,6 +266,10 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
     __asm__ volatile("" : "+v"(tid));  // as in pool_round: no per-thread addresses hoisted
     const int T = blockDim.x, N = a.N;
     const size_t stride = (size_t)P * B;
+    // A thread's dword q covers variables 4q..4q+3, whose lane bytes are 4P
+    // words apart, so for a fixed byte b the lanes of a wave fall on few LDS
+    // banks; rotating the byte order by (lane / 8) % 4 spreads them.
+    const int rot = (tid >> 3) & 3;
     for (int s = 0; s < S; s++) {
         const int io = st.io[s];
         if (!io) continue;  // uniform: read after a barrier
@@ -285,12 +297,18 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
                     if (io & 1) {
                         uint32_t x = 0;
 #pragma unroll
-                        for (int b = 0; b < 4; b++) x |= (uint32_t)((int8_t)lane[b * stride] > 0) << (8 * b);
+                        for (int b = 0; b < 4; b++) {
+                            const int bb = (b + rot) & 3;
+                            x |= (uint32_t)((int8_t)lane[bb * stride] > 0) << (8 * bb);
+                        }
                         dst[q] = x;
                     }
                     if (io & 2)
 #pragma unroll
-                        for (int b = 0; b < 4; b++) lane[b * stride] = (uint8_t)(y[r] >> (8 * b));
+                        for (int b = 0; b < 4; b++) {
+                            const int bb = (b + rot) & 3;
+                            lane[bb * stride] = (uint8_t)(y[r] >> (8 * bb));
+                        }
                 }
             }
         } else {
@@ -433,8 +451,10 @@ __device__ __forceinline__ uint16_t half_to_pair(u16x2 h) {
     return (uint16_t)__builtin_amdgcn_perm(0u, w32(h), 0x0C0C0301u);
 }
 
-template <int D, bool EQ>
-__device__ __forceinline__ uint32_t pair_check(uint16_t *post, const uint32_t *__restrict__ ev,
+// One pair-mode check whose variable ids and messages are already in
+// registers (ev[j], mr[j] for j < D); mp is where its messages go back.
+template <int D, bool EQ, int DM>
+__device__ __forceinline__ uint32_t pair_check(uint16_t *post, const uint32_t (&ev)[DM], const uint32_t (&mr)[DM],
                                                uint16_t *__restrict__ mp, int lp, int w, u16x2 keep, int variant,
                                                u16x2 lpar) {
     uint32_t vi[D];
@@ -444,7 +464,7 @@ __device__ __forceinline__ uint32_t pair_check(uint16_t *post, const uint32_t *_
 #pragma unroll
     for (int j = 0; j < D; j++) x[j] = pair_to_half(post[vi[j]]);
 #pragma unroll
-    for (int j = 0; j < D; j++) m[j] = pair_to_half(mp[j]) & keep;
+    for (int j = 0; j < D; j++) m[j] = pair_to_half(mr[j]) & keep;
     const uint32_t par = check_half<D, EQ>(x, m, false, variant, false, lpar, pn, mn);
 #pragma unroll
     for (int j = 0; j < D; j++) {
@@ -455,14 +475,14 @@ __device__ __forceinline__ uint32_t pair_check(uint16_t *post, const uint32_t *_
 }
 
 template <bool EQ, int DMAXC>
-__device__ __forceinline__ uint32_t pair_check_any(int D, uint16_t *post, const uint32_t *__restrict__ ev,
-                                                   uint16_t *__restrict__ mp, int lp, int w, u16x2 keep, int variant,
-                                                   u16x2 lpar) {
+__device__ __forceinline__ uint32_t pair_check_any(int D, uint16_t *post, const uint32_t (&ev)[DMAXC],
+                                                   const uint32_t (&mr)[DMAXC], uint16_t *__restrict__ mp, int lp,
+                                                   int w, u16x2 keep, int variant, u16x2 lpar) {
     switch (D) {
 #define LDPC_QCASE(d)                                                                        \
     case d:                                                                                  \
         if constexpr (d <= DMAXC)                                                            \
-            return pair_check<d, EQ>(post, ev, mp, lp, w, keep, variant, lpar);             \
+            return pair_check<d, EQ, DMAXC>(post, ev, mr, mp, lp, w, keep, variant, lpar);   \
         else                                                                                 \
             return 0u;
         LDPC_QCASE(1) LDPC_QCASE(2) LDPC_QCASE(3) LDPC_QCASE(4) LDPC_QCASE(5) LDPC_QCASE(6)
@@ -473,6 +493,34 @@ __device__ __forceinline__ uint32_t pair_check_any(int D, uint16_t *post, const 
         LDPC_QCASE(31) LDPC_QCASE(32)
 #undef LDPC_QCASE
     default: return 0u;
+    }
+}
+
+// Variable ids (and, with MSGS, messages) of NG checks c0, c0 + cs, ...
+// below `le`, all loads issued before any is used: one memory latency for
+// the group instead of two per check.  A missing check gets D = 0.  DMAXC
+// entries are read whatever D is (entries j >= D belong to the next check
+// and are never used); the edge-id and message arrays are padded so that the
+// last check's reads stay inside them.
+template <int NG, int DMAXC, bool MSGS>
+__device__ __forceinline__ void pair_gather(const PoolArgs &a, const uint16_t *mbase, int c0, int cs, int le, int P,
+                                            int w, uint2 (&ci)[NG], uint32_t (&ev)[NG][DMAXC],
+                                            uint32_t (&mr)[NG][DMAXC]) {
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int c = c0 + g * cs;
+        ci[g] = c < le ? a.checks[c] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int D = (int)(ci[g].x & 0xFFu);
+        const uint32_t *e = a.edge_var + ci[g].y;
+        const uint16_t *m = mbase + (size_t)ci[g].y * P + (size_t)w * D;
+#pragma unroll
+        for (int j = 0; j < DMAXC; j++) {
+            ev[g][j] = e[j];
+            if constexpr (MSGS) mr[g][j] = (uint32_t)m[j];
+        }
     }
 }
 
@@ -524,31 +572,46 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
 #pragma unroll
             for (int k = 0; k < 2; k++) rowoff[k] = min(st.it[2 * w + k], a.tits - 1) * a.tdeg;
         uint32_t ov = 0;
+        constexpr int NG = DMAXC <= 8 ? 2 : 1;  // checks in flight per thread
         for (int L = 0; L < a.n_levels; L++) {
             const int lb = a.level_beg[L], le = a.level_beg[L + 1];
-            for (int c = lb + (tid >> lp); c < le; c += cstride) {
-                const uint2 ci = a.checks[c];
-                const int D = (int)(ci.x & 0xFFu);
-                u16x2 lpar = splat(0u);
-                if (learned) {
-                    const int col = min(D, a.tdeg - 1);
-                    lpar = u2(learned_half(a.variant, s_tab[rowoff[0] + col]) |
-                              (learned_half(a.variant, s_tab[rowoff[1] + col]) << 16));
+            for (int c0 = lb + (tid >> lp); c0 < le; c0 += NG * cstride) {
+                uint2 ci[NG];
+                uint32_t ev[NG][DMAXC], mr[NG][DMAXC];
+                pair_gather<NG, DMAXC, true>(a, mbase, c0, cstride, le, P, w, ci, ev, mr);
+#pragma unroll
+                for (int g = 0; g < NG; g++) {
+                    const int D = (int)(ci[g].x & 0xFFu);
+                    if (D == 0) continue;
+                    u16x2 lpar = splat(0u);
+                    if (learned) {
+                        const int col = min(D, a.tdeg - 1);
+                        lpar = u2(learned_half(a.variant, s_tab[rowoff[0] + col]) |
+                                  (learned_half(a.variant, s_tab[rowoff[1] + col]) << 16));
+                    }
+                    ov |= pair_check_any<EQ, DMAXC>(D, post, ev[g], mr[g], mbase + (size_t)ci[g].y * P + (size_t)w * D,
+                                                    lp, w, keep, a.variant, lpar);
                 }
-                ov |= pair_check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y, mbase + (size_t)ci.y * P + (size_t)w * D,
-                                                lp, w, keep, a.variant, lpar);
             }
             __syncthreads();
         }
         if (a.stop_rule == 0) {  // syndrome of the decisions after the iteration
+            constexpr int SG = DMAXC <= 8 ? 4 : 2;
             ov = 0;
-            for (int c = tid >> lp; c < a.C; c += cstride) {
-                const uint2 ci = a.checks[c];
-                const int D = (int)(ci.x & 0xFFu);
-                const uint32_t *ev = a.edge_var + ci.y;
-                uint32_t par = 0;
-                for (int j = 0; j < D; j++) par ^= positive_msb(post[(ev[j] << lp) + (uint32_t)w]);
-                ov |= par & 0x8080u;
+            for (int c0 = tid >> lp; c0 < a.C; c0 += SG * cstride) {
+                uint2 ci[SG];
+                uint32_t ev[SG][DMAXC], unused[SG][DMAXC];
+                pair_gather<SG, DMAXC, false>(a, mbase, c0, cstride, a.C, P, w, ci, ev, unused);
+#pragma unroll
+                for (int g = 0; g < SG; g++) {
+                    const int D = (int)(ci[g].x & 0xFFu);
+                    uint32_t par = 0;
+#pragma unroll
+                    for (int j = 0; j < DMAXC; j++)
+                        if (j < D) par ^= positive_msb(post[(ev[g][j] << lp) + (uint32_t)w]);
+                    ov |= par & 0x8080u;
+                    __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
+                }
             }
         }
         if (ov) atomicOr(&s_ov[w], ov);
