@@ -113,6 +113,13 @@ __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 
     // less per edge (k1 + min1 <= 0xFE00 cannot wrap)
     const u16x2 C1 = k1 + m1;
     sg ^= splat(0x8000u);  // top bit now: parity of the others is even -> negate
+    {
+        // opaque, or the compiler re-associates (sg ^ c) ^ y into (sg ^ y) ^ c:
+        // one XOR more per edge
+        uint32_t sw = w32(sg);
+        __asm__("" : "+v"(sw));
+        sg = u2(sw);
+    }
 
 #pragma unroll
     for (int j = 0; j < D; j++) {

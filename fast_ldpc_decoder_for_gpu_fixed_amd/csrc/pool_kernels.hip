@@ -288,27 +288,42 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
                         y[r] = q < Q ? src[q] : 0u;
                     }
                 }
+                if (io & 1) {
+                    uint32_t x[IO_BATCH];
 #pragma unroll
-                for (int r = 0; r < IO_BATCH; r++) {
-                    int q = q0 + r * T;
-                    __asm__ volatile("" : "+v"(q));  // addresses formed here, one at a time
-                    if (q >= Q) break;
-                    uint8_t *lane = pb + (size_t)(4 * q) * stride + (size_t)wl * B + k;
-                    if (io & 1) {
-                        uint32_t x = 0;
+                    for (int r = 0; r < IO_BATCH; r++) {
+                        int q = q0 + r * T;
+                        __asm__ volatile("" : "+v"(q));
+                        x[r] = 0;
+                        if (q < Q) {
+                            const uint8_t *lane = pb + (size_t)(4 * q) * stride + (size_t)wl * B + k;
 #pragma unroll
-                        for (int b = 0; b < 4; b++) {
-                            const int bb = (b + rot) & 3;
-                            x |= (uint32_t)((int8_t)lane[bb * stride] > 0) << (8 * bb);
+                            for (int b = 0; b < 4; b++) {
+                                const int bb = (b + rot) & 3;
+                                x[r] |= (uint32_t)((int8_t)lane[bb * stride] > 0) << (8 * bb);
+                            }
                         }
-                        dst[q] = x;
                     }
-                    if (io & 2)
 #pragma unroll
-                        for (int b = 0; b < 4; b++) {
-                            const int bb = (b + rot) & 3;
-                            lane[bb * stride] = (uint8_t)(y[r] >> (8 * bb));
+                    for (int r = 0; r < IO_BATCH; r++) {
+                        const int q = q0 + r * T;
+                        if (q < Q) dst[q] = x[r];
+                    }
+                }
+                if (io & 2) {
+#pragma unroll
+                    for (int r = 0; r < IO_BATCH; r++) {
+                        int q = q0 + r * T;
+                        __asm__ volatile("" : "+v"(q));
+                        if (q < Q) {
+                            uint8_t *lane = pb + (size_t)(4 * q) * stride + (size_t)wl * B + k;
+#pragma unroll
+                            for (int b = 0; b < 4; b++) {
+                                const int bb = (b + rot) & 3;
+                                lane[bb * stride] = (uint8_t)(y[r] >> (8 * bb));
+                            }
                         }
+                    }
                 }
             }
         } else {
