@@ -304,7 +304,9 @@ def run_ira(L, args, t_start):
         r = counts(L, probe, d_hard.ptr, frames, K)
         alg = int(its.sum()) * (2 * E + 2 * N) + frames * 2 * N
         r.update({"info_gbps": round(frames * K / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 4),
-                  "avg_iterations": round(float(its.mean()), 4), "roofline": roofline(alg, ms, None)})
+                  "avg_iterations": round(float(its.mean()), 4),
+                  "roofline": roofline(alg, ms, traffic_for(args.traffic_json, f"ira_pool_n64800_f{frames}")
+                                       if name == "learned_pool" else None)})
         r.update(pool.config())
         legs[name] = r
     dec = L.make_decoder(code, frames // 4, L.OMS_V2, early_term=0)
