@@ -12,8 +12,7 @@
 //   v2c parity -- every check had an even parity of positive v2c values
 //                 during the iteration (the reference's rule,
 //                 src/decoder_2nms/cuda/CUDA_2NMS_SIMD.cu:237-244, per frame).
-// A
-// fresh lane reads zero messages in its first iteration: the per-word byte
+// A fresh lane reads zero messages in its first iteration: the per-word byte
 // mask `keep` clears that lane of every message word it loads.  Learned
 // tables are looked up per lane with the lane's own iteration index.
 // The workgroup exits when the queue is empty and every slot is drained.
@@ -198,6 +197,7 @@ __device__ __forceinline__ SlotState carve_slots(const PoolArgs &a, char *lds) {
 // the queue in slot order with one atomic per round; then the per-word keep
 // masks (B byte lanes per word: a refilled lane's messages are cleared) and
 // the reset of the per-word stop-test bits (`reset(q)`)
+static_assert(4 * MAX_P <= 64, "pool_round maps one slot to one lane of wave 0");
 template <int B, typename OddBit, typename Reset>
 __device__ __forceinline__ void pool_round(const PoolArgs &a, const SlotState &st, int S, int P, OddBit odd_bit,
                                            Reset reset) {
@@ -257,8 +257,10 @@ __device__ __forceinline__ void pool_round(const PoolArgs &a, const SlotState &s
 
 // every thread: decisions of the frames leaving their slots, LLRs of the
 // frames entering them (byte lane k of word wl of each variable; B = bytes
-// per lane group: 4 for words, 2 for pairs).  The LLR loads of one slot are
-// issued IO_BATCH dwords per thread at a time before any is stored.
+// per lane group: 4 for words, 2 for pairs).  Per slot and thread, IO_BATCH
+// LLR dwords are loaded first; then the leaving frame's decisions are read
+// from the lanes and stored back to back (the stores leave no register to
+// wait on); then the loaded LLRs are written into the same lanes.
 constexpr int IO_BATCH = 8;
 template <int B>
 __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, uint8_t *pb, int S, int P) {
