@@ -54,8 +54,9 @@ __device__ __forceinline__ uint32_t positive_msb(uint32_t x) {
 template <int UD>
 __device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_t *post, int lp, int w) {
     const int tid = threadIdx.x, T = blockDim.x;
+    static_assert(UD > 0, "the generic path tests its checks in the kernel (early exit)");
     uint2 synd = make_uint2(0u, 0u);
-    if constexpr (UD > 0) {
+    {
         constexpr int H = (UD + 1) / 2;
         if (a.P == 2) {
             // SB checks per thread at a time: their offsets are loaded together
@@ -100,15 +101,6 @@ __device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_
                 synd.x |= par;
             }
         }
-    } else {
-        for (int c = tid >> lp; c < a.C; c += T >> lp) {
-            const uint2 ci = a.checks[c];
-            const int D = (int)(ci.x & 0xFFu);
-            const uint32_t *ev = a.edge_var + ci.y;
-            uint32_t par = 0;
-            for (int j = 0; j < D; j++) par ^= positive_msb(post[(ev[j] << lp) + (uint32_t)w]);
-            synd.x |= par;
-        }
     }
     synd.x &= 0x80808080u;
     synd.y &= 0x80808080u;
@@ -134,6 +126,27 @@ __device__ __forceinline__ uint2 pool_check_any(int D, uint32_t *post, const uin
         LDPC_PCASE(31) LDPC_PCASE(32)
 #undef LDPC_PCASE
     default: return make_uint2(0u, 0u);
+    }
+}
+
+// Entries (degree, first edge) and variable ids of NG checks c0, c0 + cs, ...
+// below `le` (a missing check gets D = 0), all loads issued before any is
+// used.  DMAXC ids are read whatever D is (those past D belong to the next
+// check and are never used); the id array is padded so that the last
+// check's reads stay inside it.
+template <int NG, int DMAXC>
+__device__ __forceinline__ void gather_ids(const PoolArgs &a, int c0, int cs, int le, uint2 (&ci)[NG],
+                                           uint32_t (&ev)[NG][DMAXC]) {
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int c = c0 + g * cs;
+        ci[g] = c < le ? a.checks[c] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const uint32_t *e = a.edge_var + ci[g].y;
+#pragma unroll
+        for (int j = 0; j < DMAXC; j++) ev[g][j] = e[j];
     }
 }
 
@@ -429,9 +442,41 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
                 __syncthreads();
             }
         }
-        if (a.stop_rule == 0) {  // syndrome of the decisions after the iteration
+        if (a.stop_rule != 0) {
+            if (ove) atomicOr(&s_ove[w], ove);
+            if (ovo) atomicOr(&s_ovo[w], ovo);
+        } else if constexpr (UD == 0) {
+            // syndrome, generic path: as in pair mode, found failing checks
+            // are published at once and a thread stops scanning when every
+            // frame of its word already has one
+            constexpr int SG = DMAXC <= 8 ? 4 : DMAXC <= 16 ? 2 : 1;
+            uint32_t need = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (st.fid[4 * w + k] >= 0) need |= 0x80u << (8 * k);
+            uint32_t seen = 0;
+            for (int c0 = tid >> lp; c0 < a.C; c0 += SG * cstride) {
+                if ((s_ove[w] & need) == need) break;
+                uint2 ci[SG];
+                uint32_t ev[SG][DMAXC];
+                gather_ids<SG, DMAXC>(a, c0, cstride, a.C, ci, ev);
+                uint32_t found = 0;
+#pragma unroll
+                for (int g = 0; g < SG; g++) {
+                    const int D = (int)(ci[g].x & 0xFFu);
+                    uint32_t par = 0;
+#pragma unroll
+                    for (int j = 0; j < DMAXC; j++)
+                        if (j < D) par ^= positive_msb(post[(ev[g][j] << lp) + (uint32_t)w]);
+                    found |= par & 0x80808080u;
+                    __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
+                }
+                if (found & ~seen) atomicOr(&s_ove[w], found);
+                seen |= found;
+            }
+        } else {  // syndrome of the decisions after the iteration
             const uint2 sy = syndrome_items<UD>(a, post, lp, w);
-            if (UD > 0 && P == 2) {  // per check: both words; one atomic per wave and word
+            if (P == 2) {  // per check: both words; one atomic per wave and word
                 uint32_t r0 = 0, r1 = 0;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -446,9 +491,6 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
             } else if (sy.x) {
                 atomicOr(&s_ove[w], sy.x);
             }
-        } else {
-            if (ove) atomicOr(&s_ove[w], ove);
-            if (ovo) atomicOr(&s_ovo[w], ovo);
         }
         __syncthreads();
     }
@@ -513,13 +555,13 @@ __device__ __forceinline__ uint32_t pair_check_any(int D, uint16_t *post, const 
     }
 }
 
-// Variable ids (and, with MSGS, messages) of NG checks c0, c0 + cs, ...
-// below `le`, all loads issued before any is used: one memory latency for
-// the group instead of two per check.  A missing check gets D = 0.  DMAXC
+// Variable ids and 16-bit pair messages of NG checks c0, c0 + cs, ... below
+// `le`, all loads issued before any is used: one memory latency for the
+// group instead of two per check.  A missing check gets D = 0.  DMAXC
 // entries are read whatever D is (entries j >= D belong to the next check
 // and are never used); the edge-id and message arrays are padded so that the
 // last check's reads stay inside them.
-template <int NG, int DMAXC, bool MSGS>
+template <int NG, int DMAXC>
 __device__ __forceinline__ void pair_gather(const PoolArgs &a, const uint16_t *mbase, int c0, int cs, int le, int P,
                                             int w, uint2 (&ci)[NG], uint32_t (&ev)[NG][DMAXC],
                                             uint32_t (&mr)[NG][DMAXC]) {
@@ -536,7 +578,7 @@ __device__ __forceinline__ void pair_gather(const PoolArgs &a, const uint16_t *m
 #pragma unroll
         for (int j = 0; j < DMAXC; j++) {
             ev[g][j] = e[j];
-            if constexpr (MSGS) mr[g][j] = (uint32_t)m[j];
+            mr[g][j] = (uint32_t)m[j];
         }
     }
 }
@@ -595,7 +637,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
             for (int c0 = lb + (tid >> lp); c0 < le; c0 += NG * cstride) {
                 uint2 ci[NG];
                 uint32_t ev[NG][DMAXC], mr[NG][DMAXC];
-                pair_gather<NG, DMAXC, true>(a, mbase, c0, cstride, le, P, w, ci, ev, mr);
+                pair_gather<NG, DMAXC>(a, mbase, c0, cstride, le, P, w, ci, ev, mr);
 #pragma unroll
                 for (int g = 0; g < NG; g++) {
                     const int D = (int)(ci[g].x & 0xFFu);
@@ -613,12 +655,19 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
             __syncthreads();
         }
         if (a.stop_rule == 0) {  // syndrome of the decisions after the iteration
+            // The stop test only needs, per frame, whether SOME check fails:
+            // found bits are published at once, and a thread stops scanning
+            // when every frame of its pair already has a failing check (the
+            // result for frames still without one is the full scan's).
             constexpr int SG = DMAXC <= 8 ? 4 : 2;
+            const uint32_t need = (st.fid[2 * w] >= 0 ? 0x0080u : 0u) | (st.fid[2 * w + 1] >= 0 ? 0x8000u : 0u);
             ov = 0;
             for (int c0 = tid >> lp; c0 < a.C; c0 += SG * cstride) {
+                if ((s_ov[w] & need) == need) break;
                 uint2 ci[SG];
-                uint32_t ev[SG][DMAXC], unused[SG][DMAXC];
-                pair_gather<SG, DMAXC, false>(a, mbase, c0, cstride, a.C, P, w, ci, ev, unused);
+                uint32_t ev[SG][DMAXC];
+                gather_ids<SG, DMAXC>(a, c0, cstride, a.C, ci, ev);
+                uint32_t found = 0;
 #pragma unroll
                 for (int g = 0; g < SG; g++) {
                     const int D = (int)(ci[g].x & 0xFFu);
@@ -626,12 +675,15 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
 #pragma unroll
                     for (int j = 0; j < DMAXC; j++)
                         if (j < D) par ^= positive_msb(post[(ev[g][j] << lp) + (uint32_t)w]);
-                    ov |= par & 0x8080u;
+                    found |= par & 0x8080u;
                     __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
                 }
+                if (found & ~ov) atomicOr(&s_ov[w], found);
+                ov |= found;
             }
+        } else if (ov) {
+            atomicOr(&s_ov[w], ov);
         }
-        if (ov) atomicOr(&s_ov[w], ov);
         __syncthreads();
     }
 }
