@@ -97,6 +97,22 @@ def test_pool_generic_path_learned(variant, gpu_available):
     check_pool(c, variant, llr, 20, workgroups=2, uniform=0)
 
 
+@pytest.mark.parametrize("fmt", ["words", "pairs"])
+@pytest.mark.parametrize("stop_rule", [L.STOP_SYNDROME, L.STOP_V2C_PARITY], ids=["syndrome", "v2c"])
+@pytest.mark.parametrize("name", ["qc_2x4_z3", "two_checks"])
+def test_pool_tiny_codes(name, stop_rule, fmt, gpu_available):
+    """Codes with fewer edges than a check gather reads (8 entries for degrees
+    up to 8): those reads run past the last edge into the arrays' padding;
+    decisions and iteration counts stay the oracle's, on the generic word path
+    and in pair mode."""
+    c = (L.Code.qc(2, 4, 3, 0.9, seed=1) if name == "qc_2x4_z3" else
+         L.Code.from_table(6, np.array([3, 3], np.int32), np.array([0, 1, 2, 2, 3, 4], np.uint32)))
+    assert c.n_edges <= 24
+    llr = mixed_llr(c.n_vars, 600, seed=4, lo=0.0, hi=4.0)
+    tuning = {"pairs": 1} if fmt == "pairs" else {"uniform": 0}
+    check_pool(c, L.OMS_V2, llr, 12, workgroups=2, stop_rule=stop_rule, **tuning)
+
+
 def test_pool_small_queue_and_reuse(gpu_available):
     """Fewer frames than slots, then a second call on the same pool."""
     c = code("c36_1000")
