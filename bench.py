@@ -209,6 +209,9 @@ def run_matched(L, args, t_start):
                    "learned": learned, "plain": plain},
         "roofline": roofline(alg_l, ms_l, traffic_for(args.traffic_json, key)),
     }
+    tr = out["roofline"]["traffic"]
+    if tr:  # the metric's "HBM GB/s": profiled bytes per launch over this run's kernel time
+        out["config"]["hbm_gbps"] = round(tr / (ms_l * 1e-3) / 1e9, 1)
     if plain:
         out["config"]["speedup_vs_plain"] = round(gbps_l / plain["info_gbps"], 3)
         out["config"]["fer_matched"] = learned["fer"] <= plain["fer"]
