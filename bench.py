@@ -320,7 +320,7 @@ def run_ira(L, args, t_start):
                  lambda: dec.decode_device(d_words.ptr, d_hard.ptr, max_it))
     r = counts(L, dec, d_hard.ptr, frames, K)
     r.update({"info_gbps": round(frames * K / (ms_f * 1e-3) / 1e9, 3), "ms_per_step": round(ms_f, 4),
-              "avg_iterations": max_it, "decoder": "drop-in, global-memory posterior, fixed iterations",
+              "avg_iterations": max_it, "decoder": "drop-in, fixed iterations (long frames: pair-mode frame queue, no stop test)",
               "roofline": roofline(frames * (max_it * (2 * E + 2 * N) + 2 * N), ms_f, None)})
     legs["oms50_fixed"] = r
     lead = legs["learned_pool"]

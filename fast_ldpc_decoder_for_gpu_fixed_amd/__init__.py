@@ -22,7 +22,7 @@ from . import _lib
 from ._lib import LdpcError, LearnConfig, SimStats, Tuning, check, load, ptr
 
 MS, OMS, OMS_V2, NMS, NMS2, LOMS, LNMS = range(7)
-STOP_SYNDROME, STOP_V2C_PARITY = 0, 1  # frame-queue decoder stop tests
+STOP_SYNDROME, STOP_V2C_PARITY, STOP_NONE = 0, 1, 2  # frame-queue decoder stop tests
 VARIANT_NAMES = {MS: "MS", OMS: "OMS", OMS_V2: "OMS_v2", NMS: "NMS", NMS2: "2NMS",
                  LOMS: "learned-OMS", LNMS: "learned-NMS"}
 

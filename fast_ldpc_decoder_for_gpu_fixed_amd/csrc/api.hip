@@ -12,6 +12,7 @@
 #include "../../include/ldpc_mi355x.h"
 #include "code.h"
 #include "kernels.h"
+#include "pool.h"
 
 using ldpc::Plan;
 
@@ -71,6 +72,12 @@ struct ldpc_decoder {
     int P = 1, log2P = 0, threads = 256;
     int uniform_deg = 0, batch = 1;  // single-degree kernel when uniform_deg > 0
     bool post_in_lds = true;
+    // the word block does not fit LDS: fixed-iteration decodes without the
+    // per-word stop run on a pair-mode frame queue (long_pool) instead
+    bool long_frames = false;
+    ldpc_pool *long_pool = nullptr;
+    int8_t *d_frames = nullptr;   // OMS_v2 input back in frame-major order
+    std::vector<int8_t> h_table;  // the learned table, for long_pool
     int early_term = 0;
     hipStream_t stream = nullptr;
     // device
@@ -112,6 +119,8 @@ void free_decoder(ldpc_decoder *d) {
     (void)hipFree(d->d_iters);
     (void)hipFree(d->d_edge_off16);
     (void)hipFree(d->d_table);
+    (void)hipFree(d->d_frames);
+    if (d->long_pool) ldpc_pool_destroy(d->long_pool);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
@@ -289,6 +298,7 @@ ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
     const size_t post_word_bytes = (size_t)code->N * 4;
     const size_t lds_budget = ldpc::MAX_LDS_BYTES - 1024;
     d->post_in_lds = tu.post_in_lds >= 0 ? (tu.post_in_lds != 0) : (post_word_bytes <= lds_budget);
+    d->long_frames = tu.post_in_lds < 0 && post_word_bytes > lds_budget;
     if (d->post_in_lds && post_word_bytes > lds_budget) {
         free_decoder(d);
         fail(LDPC_ERR_ARG, "posterior of one word does not fit in LDS");
@@ -429,16 +439,56 @@ int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, 
     HIP_TRY(hipMemcpy(dec->d_table, table, (size_t)n_iters * n_degs, hipMemcpyHostToDevice));
     dec->tits = n_iters;
     dec->tdeg = n_degs;
+    dec->h_table.assign(table, table + (size_t)n_iters * n_degs);
+    if (dec->long_pool) return ldpc_pool_set_table(dec->long_pool, table, n_iters, n_degs);
     return LDPC_OK;
 }
 
 static size_t io_bytes(const ldpc_decoder *d) { return (size_t)d->code.N * d->W * 4; }
+
+// Long frames (word block larger than LDS), fixed iterations, no per-word
+// stop: each frame is decoded for exactly `iterations` iterations by the
+// pair-mode frame queue (two frames per 16-bit lane, posterior in LDS) with
+// no stop test -- the same per-frame layered decode, hence the same
+// decisions as the global-memory-posterior kernel.  Not for OMS with two
+// degree groups (its first-iteration rule is per word group, not per frame).
+static bool use_long_pool(const ldpc_decoder *d) {
+    return d->long_frames && !d->early_term && !(d->variant == LDPC_OMS && d->code.q1_split < d->code.C);
+}
+static int decode_long(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d_out, int iterations, hipStream_t s) {
+    if ((dec->variant == LDPC_LOMS || dec->variant == LDPC_LNMS) && dec->h_table.empty())
+        return fail(LDPC_ERR_STATE, "learned variant needs ldpc_decoder_set_table first");
+    if (!dec->long_pool) {
+        const ldpc_tuning tu = {0, 0, -1, 2, 0, 0};  // pairs, automatic block and workgroup size
+        dec->long_pool = ldpc_pool_create(&dec->code, dec->variant, iterations, LDPC_STOP_NONE, &tu);
+        if (!dec->long_pool) return LDPC_ERR_ARG;  // (the caller falls back to the global-memory kernel)
+        if (!dec->h_table.empty()) {
+            const int rc = ldpc_pool_set_table(dec->long_pool, dec->h_table.data(), dec->tits, dec->tdeg);
+            if (rc) return rc;
+        }
+    }
+    dec->long_pool->max_iters = iterations;
+    const int8_t *frames = d_in;
+    if (dec->variant == LDPC_OMS_V2) {  // input in the device word layout
+        if (!dec->d_frames) HIP_TRY(dalloc(&dec->d_frames, io_bytes(dec)));
+        HIP_TRY(ldpc::launch_deinterleave_raw(reinterpret_cast<const uint32_t *>(d_in), dec->d_frames, dec->code.N,
+                                              (int)dec->W, s));
+        frames = dec->d_frames;
+    }
+    return ldpc_pool_decode(dec->long_pool, frames, 4 * (uint64_t)dec->W, d_out, nullptr, 0, s);
+}
 
 int ldpc_decoder_decode_device(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d_out, int iterations, void *stream) {
     if (!dec || !d_in || !d_out) return fail(LDPC_ERR_ARG, "null argument");
     int rc = check_iters(iterations);
     if (rc) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : dec->stream;
+    if (use_long_pool(dec)) {
+        dec->loaded = false;
+        rc = decode_long(dec, d_in, d_out, iterations, s);
+        if (rc != LDPC_ERR_ARG || dec->long_pool) return rc;
+        dec->long_frames = false;  // no pair block fits either: global-memory posterior
+    }
     if (dec->early_term) {
         rc = reset_counters(dec, s);
         if (rc) return rc;

@@ -287,19 +287,25 @@ __global__ __launch_bounds__(256) void k_interleave(const int8_t *__restrict__ i
     }
 }
 
-__global__ __launch_bounds__(256) void k_deinterleave_hd(const uint32_t *__restrict__ in,
-                                                         uint8_t *__restrict__ out, int N, int W) {
+// HD: the hard decision of each byte (the reference's inverse interleave);
+// else the bytes themselves (device words back to frame-major LLRs)
+template <bool HD>
+__global__ __launch_bounds__(256) void k_deinterleave(const uint32_t *__restrict__ in,
+                                                      uint8_t *__restrict__ out, int N, int W) {
     __shared__ uint32_t tile[TV][TW + 1];
     const int v0 = blockIdx.x * TV, w0 = blockIdx.y * TW, tid = threadIdx.x;
     for (int i = tid; i < TV * TW; i += 256) {
         const int vl = i / TW, wl = i % TW;
         uint32_t x = 0;
         if (v0 + vl < N && w0 + wl < W) x = in[(size_t)(v0 + vl) * W + w0 + wl];
-        // vsetgts4(x, 0): byte -> 1 if > 0 (GPU_Transpose_uint8.cu:29)
-        uint32_t hd = 0;
+        if constexpr (HD) {
+            // vsetgts4(x, 0): byte -> 1 if > 0 (GPU_Transpose_uint8.cu:29)
+            uint32_t hd = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) hd |= (uint32_t)((int8_t)(x >> (8 * k)) > 0) << (8 * k);
-        tile[vl][wl] = hd;
+            for (int k = 0; k < 4; k++) hd |= (uint32_t)((int8_t)(x >> (8 * k)) > 0) << (8 * k);
+            x = hd;
+        }
+        tile[vl][wl] = x;
     }
     __syncthreads();
     const uint8_t *tb = reinterpret_cast<const uint8_t *>(&tile[0][0]);
@@ -394,7 +400,13 @@ hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipS
 
 hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s) {
     dim3 g((unsigned)((N + TV - 1) / TV), (unsigned)((W + TW - 1) / TW));
-    hipLaunchKernelGGL(k_deinterleave_hd, g, dim3(256), 0, s, in, out, N, W);
+    hipLaunchKernelGGL(k_deinterleave<true>, g, dim3(256), 0, s, in, out, N, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_deinterleave_raw(const uint32_t *in, int8_t *out, int N, int W, hipStream_t s) {
+    dim3 g((unsigned)((N + TV - 1) / TV), (unsigned)((W + TW - 1) / TW));
+    hipLaunchKernelGGL(k_deinterleave<false>, g, dim3(256), 0, s, in, reinterpret_cast<uint8_t *>(out), N, W);
     return hipGetLastError();
 }
 

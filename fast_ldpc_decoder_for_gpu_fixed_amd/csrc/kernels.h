@@ -65,6 +65,7 @@ bool uniform_degree_supported(int d);
 int max_threads_for_degree(int dmax, bool uniform);
 hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipStream_t s);
 hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s);
+hipError_t launch_deinterleave_raw(const uint32_t *in, int8_t *out, int N, int W, hipStream_t s);
 
 // persistent frame-queue decoder (pool_kernels.hip)
 struct PoolArgs {

@@ -90,7 +90,7 @@ extern "C" {
 ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations, int stop_rule,
                             const ldpc_tuning *tuning) {
     if (!code || max_iterations < 1 || variant < LDPC_MS || variant > LDPC_LNMS ||
-        (stop_rule != LDPC_STOP_SYNDROME && stop_rule != LDPC_STOP_V2C_PARITY)) {
+        (stop_rule != LDPC_STOP_SYNDROME && stop_rule != LDPC_STOP_V2C_PARITY && stop_rule != LDPC_STOP_NONE)) {
         ldpc::set_last_error("ldpc_pool_create: bad arguments");
         return nullptr;
     }

@@ -211,6 +211,7 @@ __device__ __forceinline__ SlotState carve_slots(const PoolArgs &a, char *lds) {
 // masks (B byte lanes per word: a refilled lane's messages are cleared) and
 // the reset of the per-word stop-test bits (`reset(q)`)
 static_assert(4 * MAX_P <= 64, "pool_round maps one slot to one lane of wave 0");
+constexpr int STOP_NONE = 2;  // LDPC_STOP_NONE: every frame runs max_iters
 template <int B, typename OddBit, typename Reset>
 __device__ __forceinline__ void pool_round(const PoolArgs &a, const SlotState &st, int S, int P, OddBit odd_bit,
                                            Reset reset) {
@@ -382,6 +383,7 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
                 a, st, S, P,
                 [&](int s) {
                     const int wl = s >> 2, k = s & 3;
+                    if (a.stop_rule == STOP_NONE) return true;
                     if (a.stop_rule == 0) return (bool)((s_ove[wl] >> (8 * k + 7)) & 1u);
                     const uint32_t src = (k & 1) ? s_ovo[wl] : s_ove[wl];
                     return (bool)((src >> ((k & 2) ? 31 : 15)) & 1u);
@@ -442,7 +444,9 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
                 __syncthreads();
             }
         }
-        if (a.stop_rule != 0) {
+        if (a.stop_rule == STOP_NONE) {
+            // no stop test: the frame runs to max_iters
+        } else if (a.stop_rule != 0) {
             if (ove) atomicOr(&s_ove[w], ove);
             if (ovo) atomicOr(&s_ovo[w], ovo);
         } else if constexpr (UD == 0) {
@@ -614,6 +618,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
             pool_round<2>(
                 a, st, S, P,
                 [&](int s) {
+                    if (a.stop_rule == STOP_NONE) return true;
                     const uint32_t bit = a.stop_rule == 0 ? (8u * (s & 1) + 7u) : ((s & 1) ? 31u : 15u);
                     return (bool)((s_ov[s >> 1] >> bit) & 1u);
                 },
@@ -681,7 +686,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                 if (found & ~ov) atomicOr(&s_ov[w], found);
                 ov |= found;
             }
-        } else if (ov) {
+        } else if (ov && a.stop_rule != STOP_NONE) {
             atomicOr(&s_ov[w], ov);
         }
         __syncthreads();

@@ -135,7 +135,12 @@ void ldpc_decoder_destroy(ldpc_decoder *dec);
  * in: 4W*N int8 (frame-major; var-major words for OMS_v2); out: 4W*N bytes. */
 int ldpc_decoder_decode(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations);
 /* Same on device buffers, enqueued on `stream` (NULL = the decoder's own).
- * Asynchronous: returns after the launch. */
+ * Asynchronous: returns after the launch.  Frames whose 4-frame word block
+ * does not fit in LDS (and were left to the automatic choice) are decoded,
+ * when the iterations are fixed, two per 16-bit lane by the frame-queue
+ * kernel with no stop test (same decisions, posterior in LDS); OMS on codes
+ * with two degree groups and the per-word early stop use the global-memory
+ * posterior.  Calls on one decoder must not overlap in that case. */
 int ldpc_decoder_decode_device(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d_out,
                                int iterations, void *stream);
 
@@ -234,7 +239,9 @@ int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, u
  *   LDPC_STOP_SYNDROME    the decisions (post > 0) satisfy every check;
  *   LDPC_STOP_V2C_PARITY  every check had an even parity of positive v2c
  *                         values during the iteration (the reference's rule,
- *                         applied to each frame).
+ *                         applied to each frame);
+ *   LDPC_STOP_NONE        none: every frame runs max_iterations (its
+ *                         decisions equal the fixed-iteration decoder's).
  * Variants: MS, OMS_V2, NMS, 2NMS, LOMS, LNMS, and OMS on codes with a single
  * degree group (the OMS first-iteration quirk is not per-frame).  Results
  * per frame equal a one-frame layered decode with that stop test.
@@ -242,7 +249,7 @@ int ldpc_simulate(ldpc_decoder *dec, double ebn0_db, int quant, uint64_t seed, u
  * bytes of LDS per workgroup), 2 = pairs of frames (N * 2 * P bytes; for
  * long frames such as N = 64800 whose word block does not fit), -1 = words
  * when they fit, else pairs.  words_per_block then counts words or pairs. */
-enum { LDPC_STOP_SYNDROME = 0, LDPC_STOP_V2C_PARITY = 1 };
+enum { LDPC_STOP_SYNDROME = 0, LDPC_STOP_V2C_PARITY = 1, LDPC_STOP_NONE = 2 };
 typedef struct ldpc_pool ldpc_pool;
 ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations, int stop_rule,
                             const ldpc_tuning *tuning);
