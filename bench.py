@@ -277,6 +277,9 @@ def run_plain(L, args, t_start):
         "roofline": roofline(alg, ms, traffic_for(args.traffic_json, f"oms_3_6_n4000_f{frames}_it{args.iters}")
                              if variant == L.OMS else None),
     }
+    tr = out["roofline"]["traffic"]
+    if tr:  # profiled HBM bytes per launch over this run's kernel time
+        out["config"]["hbm_gbps"] = round(tr / (ms * 1e-3) / 1e9, 1)
     if not args.no_cpu_baseline and variant == L.OMS:
         n = 4 * args.cpu_baseline_words * N
         llr = d_in.download(np.empty(n, np.int8))
