@@ -3,6 +3,7 @@ compare with plain OMS on identical frames.
 
     python -m fast_ldpc_decoder_for_gpu_fixed_amd.learned [--steps 256 --batch 4096 ...]
     python -m fast_ldpc_decoder_for_gpu_fixed_amd.learned --code ira     # the N=64800 IRA code
+    python -m fast_ldpc_decoder_for_gpu_fixed_amd.learned --eval-table tables/loms_3_6_n4000.json  # evaluate only
 
 IRA: the table [iteration][check degree] is trained on a 1/10-length member of
 the same ensemble (N=6480, same degree profile: the float shadow's posterior
@@ -137,6 +138,8 @@ def main(argv=None):
     ap.add_argument("--eval-seed", type=int, default=777)
     ap.add_argument("--save-table", default="", help="write the quantised table and its training record (JSON)")
     ap.add_argument("--no-eval", action="store_true")
+    ap.add_argument("--eval-table", default="",
+                    help="evaluate a table written with --save-table instead of training one")
     ap.add_argument("--drop-last-row", type=int, default=None,
                     help="train one extra unrolled iteration and drop its row (its entry only sharpens the final "
                          "output, so it learns the loss's reward for magnitude rather than decoding; default: on "
@@ -158,6 +161,16 @@ def main(argv=None):
     else:
         code = eval_code = code_3_6(4000, seed=1)
         desc = "(3,6) N=4000 K=2000, colour-ordered table, seed 1"
+    col = code.max_deg
+    if a.eval_table:
+        variant, table = load_table(a.eval_table)
+        iters = table.shape[0]
+        res = {"code": desc, "table": a.eval_table, "int8_table_deg%d" % col: [int(x) for x in table[:, col]]}
+        res["evaluation"] = (evaluate_ira(eval_code, table, variant, 50, eval_ebn0, eval_frames, a.eval_seed) if ira
+                             else evaluate(code, table, variant, iters, eval_ebn0, eval_frames, a.eval_seed))
+        json.dump(res, sys.stdout, indent=1)
+        print()
+        return
     mode = LEARN_OFFSET if a.mode == "offset" else LEARN_SCALE
     params, hist, secs = train(code, iters + drop, batch, a.steps, a.lr, (lo, hi), a.seed, mode, quant=quant)
     params = params[:iters]
@@ -165,7 +178,6 @@ def main(argv=None):
         table, variant = quantize_offsets(params), LOMS
     else:
         table, variant = quantize_scales(params), LNMS
-    col = code.max_deg
     res = {
         "code": desc,
         "training": {"mode": a.mode, "iterations": iters, "batch_frames": batch, "steps": a.steps,
