@@ -88,8 +88,10 @@ struct PoolArgs {
     int tits, tdeg;
     const uint32_t *edge_off16;   // single-degree path: packed LDS byte offsets, or null
     int uniform_deg, batch;       // degree of that path, items in flight per thread (1, 2, 4)
-    int stop_rule;                // 0: syndrome of the decisions, 1: v2c parity (reference rule)
+    int stop_rule;                // 0: syndrome of the decisions, 1: v2c parity (reference rule), 2: none
     int pairs;                    // 1: posterior and messages hold 2 frames per 16 bits (P = pairs)
+    const uint32_t *rows;         // pair mode, degrees <= 8: [C][8] variable ids per check (unused
+                                  // entries 0), and messages as [grid][C][P][8] u16 runs; else null
 };
 // largest posterior block of a frame-queue workgroup; the slot bookkeeping and
 // the learned table (< 4.4 KB) follow it in the same dynamic LDS allocation

@@ -22,6 +22,7 @@ struct ldpc_pool {
     int uniform_deg = 0, batch = 1;
     int stop_rule = 0;
     int pairs = 0;  // posterior and messages as 2-frame pairs (P counts pairs)
+    uint32_t *d_rows = nullptr;  // pair mode with degrees <= 8: ids as rows of 8 per check
     int n_cu = 1;   // compute units of the device
 };
 

@@ -34,6 +34,7 @@ void pool_free(ldpc_pool *p) {
     (void)hipFree(p->d_next);
     (void)hipFree(p->d_table);
     (void)hipFree(p->d_edge_off16);
+    (void)hipFree(p->d_rows);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -66,6 +67,7 @@ PoolArgs pool_args(const ldpc_pool *p) {
     a.batch = p->batch;
     a.stop_rule = p->stop_rule;
     a.pairs = p->pairs;
+    a.rows = p->d_rows;
     return a;
 }
 
@@ -196,10 +198,22 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     auto al = [&](auto **q, size_t n) {
         if (e == hipSuccess && n) e = hipMalloc((void **)q, n * sizeof(**q));
     };
+    // pair mode with degrees <= 8: ids as rows of 8 per check, messages as
+    // runs of 8 u16 per check and pair (16-byte aligned, addressed by the check)
+    std::vector<uint32_t> rows;
+    if (p->pairs && code->max_deg <= 8) {
+        rows.assign((size_t)code->C * 8, 0u);
+        for (int c = 0; c < code->C; c++) {
+            const int D = (int)(p->plan.check_info[c] & 0xFFu);
+            for (int j = 0; j < D; j++) rows[(size_t)c * 8 + j] = p->plan.edge_var[p->plan.check_e0[c] + j];
+        }
+    }
+    const size_t msg_u16 = rows.empty() ? (size_t)code->E : (size_t)code->C * 8;  // per frame group
     al(&p->d_checks, ck.size());
     al(&p->d_edge_var, (size_t)code->E + ldpc::GATHER_PAD);
     al(&p->d_level_beg, p->plan.level_beg.size());
-    al(&p->d_msgs, ((size_t)p->max_grid * code->E * P * wb / code->N + 3) / 4 + ldpc::GATHER_PAD);  // u32 units
+    al(&p->d_msgs, ((size_t)p->max_grid * msg_u16 * P * wb / code->N + 3) / 4 + ldpc::GATHER_PAD);  // u32 units
+    al(&p->d_rows, rows.size());
     al(&p->d_next, 1);
     al(&p->d_edge_off16, off16.size());
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->stream, hipStreamDefault);
@@ -211,6 +225,7 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
                       hipMemcpyHostToDevice);
     if (e == hipSuccess && !off16.empty())
         e = hipMemcpy(p->d_edge_off16, off16.data(), off16.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !rows.empty()) e = hipMemcpy(p->d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         ldpc::set_last_error(std::string("ldpc_pool_create: ") + hipGetErrorString(e));
         pool_free(p);

@@ -587,6 +587,88 @@ __device__ __forceinline__ void pair_gather(const PoolArgs &a, const uint16_t *m
     }
 }
 
+// Pair mode with degrees <= 8 (rows): a check's variable ids are one row of 8
+// and its messages one 16-byte run of 8 u16 per pair, [workgroup][check]
+// [pair][8], both addressed by the check index alone: two 16-byte loads for
+// the ids, one load and one store for the messages (entries past D unused).
+template <int NG>
+__device__ __forceinline__ void rows_gather(const PoolArgs &a, const uint16_t *mbase, int c0, int cs, int le, int P,
+                                            int w, uint2 (&ci)[NG], uint32_t (&ev)[NG][8], uint32_t (&mr)[NG][8]) {
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int c = c0 + g * cs;
+        const bool ok = c < le;
+        const int cc = ok ? c : 0;
+        ci[g] = ok ? a.checks[c] : make_uint2(0u, 0u);
+        const uint4 *r = reinterpret_cast<const uint4 *>(a.rows + (size_t)cc * 8);
+        const uint4 r0 = r[0], r1 = r[1];
+        ev[g][0] = r0.x, ev[g][1] = r0.y, ev[g][2] = r0.z, ev[g][3] = r0.w;
+        ev[g][4] = r1.x, ev[g][5] = r1.y, ev[g][6] = r1.z, ev[g][7] = r1.w;
+        const uint4 m = *reinterpret_cast<const uint4 *>(mbase + ((size_t)cc * P + w) * 8);
+        const uint32_t mw[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            mr[g][2 * k] = mw[k] & 0xFFFFu;
+            mr[g][2 * k + 1] = mw[k] >> 16;
+        }
+    }
+}
+template <int NG>
+__device__ __forceinline__ void rows_ids(const PoolArgs &a, int c0, int cs, int le, uint2 (&ci)[NG],
+                                         uint32_t (&ev)[NG][8]) {
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int c = c0 + g * cs;
+        const bool ok = c < le;
+        ci[g] = ok ? a.checks[c] : make_uint2(0u, 0u);
+        const uint4 *r = reinterpret_cast<const uint4 *>(a.rows + (size_t)(ok ? c : 0) * 8);
+        const uint4 r0 = r[0], r1 = r[1];
+        ev[g][0] = r0.x, ev[g][1] = r0.y, ev[g][2] = r0.z, ev[g][3] = r0.w;
+        ev[g][4] = r1.x, ev[g][5] = r1.y, ev[g][6] = r1.z, ev[g][7] = r1.w;
+    }
+}
+// one rows-layout check: as pair_check, the 8 messages stored as one run
+// (two halves' 16-bit pairs per dword: bytes 1 and 3 of each half)
+template <int D, bool EQ>
+__device__ __forceinline__ uint32_t rows_check(uint16_t *post, const uint32_t (&ev)[8], const uint32_t (&mr)[8],
+                                               uint4 *__restrict__ mp, int lp, int w, u16x2 keep, int variant,
+                                               u16x2 lpar) {
+    uint32_t vi[D];
+    u16x2 x[D], m[D], pn[D], mn[D];
+#pragma unroll
+    for (int j = 0; j < D; j++) vi[j] = (ev[j] << lp) + (uint32_t)w;
+#pragma unroll
+    for (int j = 0; j < D; j++) x[j] = pair_to_half(post[vi[j]]);
+#pragma unroll
+    for (int j = 0; j < D; j++) m[j] = pair_to_half(mr[j]) & keep;
+    const uint32_t par = check_half<D, EQ>(x, m, false, variant, false, lpar, pn, mn);
+    uint32_t h[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) h[j] = j < D ? w32(mn[j]) : 0u;
+    uint4 o;
+    o.x = __builtin_amdgcn_perm(h[1], h[0], 0x07050301u);
+    o.y = __builtin_amdgcn_perm(h[3], h[2], 0x07050301u);
+    o.z = __builtin_amdgcn_perm(h[5], h[4], 0x07050301u);
+    o.w = __builtin_amdgcn_perm(h[7], h[6], 0x07050301u);
+    *mp = o;
+#pragma unroll
+    for (int j = 0; j < D; j++) post[vi[j]] = half_to_pair(pn[j]);
+    return par;
+}
+template <bool EQ>
+__device__ __forceinline__ uint32_t rows_check_any(int D, uint16_t *post, const uint32_t (&ev)[8],
+                                                   const uint32_t (&mr)[8], uint4 *__restrict__ mp, int lp, int w,
+                                                   u16x2 keep, int variant, u16x2 lpar) {
+    switch (D) {
+#define LDPC_RCASE(d) \
+    case d: return rows_check<d, EQ>(post, ev, mr, mp, lp, w, keep, variant, lpar);
+        LDPC_RCASE(1) LDPC_RCASE(2) LDPC_RCASE(3) LDPC_RCASE(4) LDPC_RCASE(5) LDPC_RCASE(6) LDPC_RCASE(7)
+        LDPC_RCASE(8)
+#undef LDPC_RCASE
+    default: return 0u;
+    }
+}
+
 template <bool EQ, int DMAXC>
 __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const PoolArgs a) {
     extern __shared__ uint16_t lds_post16[];
@@ -600,7 +682,8 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
     const int w = tid & (P - 1);
     const int cstride = T >> lp;
     uint16_t *post = lds_post16;
-    uint16_t *mbase = reinterpret_cast<uint16_t *>(a.msgs) + (size_t)blockIdx.x * a.E * P;
+    constexpr bool ROWS = DMAXC == 8;  // degrees <= 8: the rows layout (the host provides a.rows)
+    uint16_t *mbase = reinterpret_cast<uint16_t *>(a.msgs) + (size_t)blockIdx.x * (ROWS ? (size_t)a.C * 8 : a.E) * P;
     uint8_t *pb = reinterpret_cast<uint8_t *>(post);
     const bool learned = a.ltable != nullptr;
     if (learned)
@@ -642,7 +725,10 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
             for (int c0 = lb + (tid >> lp); c0 < le; c0 += NG * cstride) {
                 uint2 ci[NG];
                 uint32_t ev[NG][DMAXC], mr[NG][DMAXC];
-                pair_gather<NG, DMAXC>(a, mbase, c0, cstride, le, P, w, ci, ev, mr);
+                if constexpr (ROWS)
+                    rows_gather<NG>(a, mbase, c0, cstride, le, P, w, ci, ev, mr);
+                else
+                    pair_gather<NG, DMAXC>(a, mbase, c0, cstride, le, P, w, ci, ev, mr);
 #pragma unroll
                 for (int g = 0; g < NG; g++) {
                     const int D = (int)(ci[g].x & 0xFFu);
@@ -653,8 +739,14 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                         lpar = u2(learned_half(a.variant, s_tab[rowoff[0] + col]) |
                                   (learned_half(a.variant, s_tab[rowoff[1] + col]) << 16));
                     }
-                    ov |= pair_check_any<EQ, DMAXC>(D, post, ev[g], mr[g], mbase + (size_t)ci[g].y * P + (size_t)w * D,
-                                                    lp, w, keep, a.variant, lpar);
+                    if constexpr (ROWS)
+                        ov |= rows_check_any<EQ>(D, post, ev[g], mr[g],
+                                                 reinterpret_cast<uint4 *>(mbase + ((size_t)(c0 + g * cstride) * P + w) * 8),
+                                                 lp, w, keep, a.variant, lpar);
+                    else
+                        ov |= pair_check_any<EQ, DMAXC>(D, post, ev[g], mr[g],
+                                                        mbase + (size_t)ci[g].y * P + (size_t)w * D, lp, w, keep,
+                                                        a.variant, lpar);
                 }
             }
             __syncthreads();
@@ -671,7 +763,10 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                 if ((s_ov[w] & need) == need) break;
                 uint2 ci[SG];
                 uint32_t ev[SG][DMAXC];
-                gather_ids<SG, DMAXC>(a, c0, cstride, a.C, ci, ev);
+                if constexpr (ROWS)
+                    rows_ids<SG>(a, c0, cstride, a.C, ci, ev);
+                else
+                    gather_ids<SG, DMAXC>(a, c0, cstride, a.C, ci, ev);
                 uint32_t found = 0;
 #pragma unroll
                 for (int g = 0; g < SG; g++) {
@@ -733,6 +828,7 @@ hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipSt
     if (dmax < 1 || dmax > MAX_DEG || grid < 1 || threads > max_threads_for_degree(dmax, a.uniform_deg > 0))
         return hipErrorInvalidValue;
     if (a.uniform_deg && !a.edge_off16) return hipErrorInvalidValue;  // the path reads its offset table
+    if (a.pairs && dmax <= 8 && !a.rows) return hipErrorInvalidValue;  // pair mode, degrees <= 8: rows
     const void *f = pool_kernel(a, dmax);
     if (!f) return hipErrorInvalidValue;
     void *args[] = {const_cast<PoolArgs *>(&a)};
