@@ -160,8 +160,11 @@ __device__ void store_block(const DecodeArgs &a, const uint32_t *post, size_t b)
 // ---------------------------------------------------------------------------
 // the decode kernel: one workgroup per block of P words, all iterations
 // ---------------------------------------------------------------------------
+// The generic path for degrees <= 16 is held to 168 VGPRs (3 waves per SIMD;
+// with the posterior in global memory it needs 169 unconstrained).
 template <bool EQ, bool POST_IN_LDS, int DMAXC, int UD, int G, int VK>
-__global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_decode(const DecodeArgs a) {
+__global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256))
+__attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 16 ? 3 : 1))) void k_decode(const DecodeArgs a) {
     extern __shared__ uint32_t lds_post[];
     __shared__ uint32_t s_ov[MAX_P];
     __shared__ int s_active[MAX_P];

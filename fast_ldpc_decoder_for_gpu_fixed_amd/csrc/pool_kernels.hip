@@ -353,7 +353,12 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
 }
 
 template <bool EQ, int DMAXC, int UD, int G, int VK>
-__global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_pool(const PoolArgs a) {
+// Register budgets of the generic path: degrees <= 10 fit 128 VGPRs (4 waves
+// per SIMD, 125 used), degrees <= 16 are held to 168 (3 waves instead of the
+// 2 that 176 unconstrained registers give); neither spills.
+__global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256))
+__attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 10 ? 4 : UD == 0 && DMAXC == 16 ? 3 : 1))) void k_pool(
+    const PoolArgs a) {
     extern __shared__ uint32_t lds_post[];
     const SlotState st = carve_slots(a, reinterpret_cast<char *>(lds_post));
     uint32_t *const s_keep = st.keep, *const s_ove = st.ove, *const s_ovo = st.ovo;
@@ -397,12 +402,12 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
 
         // ---- one layered iteration over all slots
         const uint32_t keep = s_keep[w];
-        int rowoff[4] = {0, 0, 0, 0};
-        if (learned)
-#pragma unroll
-            for (int k = 0; k < 4; k++) rowoff[k] = min(st.it[4 * w + k], a.tits - 1) * a.tdeg;
         uint32_t ove = 0, ovo = 0;
         if constexpr (UD > 0) {
+            int rowoff[4] = {0, 0, 0, 0};
+            if (learned)
+#pragma unroll
+                for (int k = 0; k < 4; k++) rowoff[k] = min(st.it[4 * w + k], a.tits - 1) * a.tdeg;
             u16x2 lpe = splat(0u), lpo = splat(0u);
             if (learned) {
                 const int col = min(UD, a.tdeg - 1);
@@ -427,13 +432,14 @@ __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256)) void k_po
                     const int D = (int)(ci.x & 0xFFu);
                     u16x2 lpe = splat(0u), lpo = splat(0u);
                     if (learned) {
+                        // the lanes' table rows re-read here: no registers held across the level loop
                         const int col = min(D, a.tdeg - 1);
-                        const uint32_t l0 = learned_half(a.variant, s_tab[rowoff[0] + col]);
-                        const uint32_t l1 = learned_half(a.variant, s_tab[rowoff[1] + col]);
-                        const uint32_t l2 = learned_half(a.variant, s_tab[rowoff[2] + col]);
-                        const uint32_t l3 = learned_half(a.variant, s_tab[rowoff[3] + col]);
-                        lpe = u2(l0 | (l2 << 16));
-                        lpo = u2(l1 | (l3 << 16));
+                        uint32_t l[4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            l[k] = learned_half(a.variant, s_tab[min(st.it[4 * w + k], a.tits - 1) * a.tdeg + col]);
+                        lpe = u2(l[0] | (l[2] << 16));
+                        lpo = u2(l[1] | (l[3] << 16));
                     }
                     const uint2 pr = pool_check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y,
                                                                mbase + (size_t)ci.y * P + (size_t)w * D, lp, w, keep,
@@ -820,6 +826,7 @@ static const void *pool_kernel(const PoolArgs &a, int dmax) {
         return uniform_kernel<false, -1>(a.uniform_deg, a.batch);
     }
     if (dmax <= 8) return eq ? (const void *)k_pool<true, 8, 0, 1, -1> : (const void *)k_pool<false, 8, 0, 1, -1>;
+    if (dmax <= 10) return eq ? (const void *)k_pool<true, 10, 0, 1, -1> : (const void *)k_pool<false, 10, 0, 1, -1>;
     if (dmax <= 16) return eq ? (const void *)k_pool<true, 16, 0, 1, -1> : (const void *)k_pool<false, 16, 0, 1, -1>;
     return eq ? (const void *)k_pool<true, 32, 0, 1, -1> : (const void *)k_pool<false, 32, 0, 1, -1>;
 }

@@ -213,6 +213,12 @@ def test_global_posterior_path(gpu_available):
     llr = channel_llr(4000, 4 * W, seed=77)
     for variant in (L.OMS, L.NMS2):
         run_lockstep(code, W, variant, llr, 5, post_in_lds=0, words_per_block=4)
+    # degrees 10..16: the generic kernel of that class, posterior in global memory
+    qc = L.Code.qc(4, 24, 96, 0.35, seed=2)
+    assert 8 < qc.max_deg <= 16
+    llr = channel_llr(qc.n_vars, 4 * W, ebn0=3.5, rate=1.0 - qc.n_checks / qc.n_vars, seed=78)
+    for variant in (L.OMS_V2, L.NMS2):
+        run_lockstep(qc, W, variant, llr, 5, post_in_lds=0, words_per_block=4)
 
 
 @pytest.mark.parametrize("variant", [L.OMS, L.MS, L.NMS, L.NMS2, L.OMS_V2], ids=lambda v: L.VARIANT_NAMES[v])

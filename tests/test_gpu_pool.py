@@ -54,6 +54,7 @@ def code(name):
             "c36_1002": lambda: L.code_3_6(1002, seed=4),
             "qc_4x24_z24": lambda: L.Code.qc(4, 24, 24, 0.9, seed=2),
             "qc_12x24_z27": lambda: L.Code.qc(12, 24, 27, 0.9, seed=2),
+            "qc_12x24_z27_d9": lambda: L.Code.qc(12, 24, 27, 0.35, seed=3),  # degrees 3..9
         }[name]()
     return _CODES[name]
 
@@ -68,7 +69,7 @@ def test_pool_parity_refills(variant, stop_rule, gpu_available):
     assert (its < 20).any()
 
 
-@pytest.mark.parametrize("name", ["c36_1002", "qc_4x24_z24", "qc_12x24_z27"])
+@pytest.mark.parametrize("name", ["c36_1002", "qc_4x24_z24", "qc_12x24_z27", "qc_12x24_z27_d9"])
 @pytest.mark.parametrize("variant", [L.OMS_V2, L.NMS2, L.LOMS])
 def test_pool_parity_codes(name, variant, gpu_available):
     c = code(name)
