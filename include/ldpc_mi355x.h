@@ -259,7 +259,8 @@ int ldpc_pool_set_table(ldpc_pool *pool, const int8_t *table, int n_iters, int n
  * stay resident on the GPU (capped by the work).  Asynchronous on `stream`
  * (NULL = the pool's own); the queue counter is reset with a kernel, so the
  * call may be captured in a HIP graph.  Calls on one pool must not overlap:
- * they share its queue head and message buffer (use one pool per stream). */
+ * they share its queue head and message buffer (use one pool per stream).
+ * frames = 0 is LDPC_ERR_ARG (as is an empty step of ldpc_mixed_decode). */
 int ldpc_pool_decode(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters,
                      int workgroups, void *stream);
 /* One Eb/N0 point: device channel (frames 0 .. n_frames-1 of the stream

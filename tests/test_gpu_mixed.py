@@ -139,6 +139,8 @@ def test_mixed_unknown_code_id_is_skipped(gpu_available):
     keep = cid < len(cs)
     ref_h, ref_i = oracle_mixed(cs, np.where(keep, cid, 255).astype(np.uint8), off, buf, L.MS, 15)
     np.testing.assert_array_equal(its[keep], ref_i[keep])
+    with pytest.raises(L.LdpcError):  # an empty step
+        mixed.decode_device(b.d_in.ptr, b.d_off.ptr, b.d_cid.ptr, 0, b.d_out.ptr)
 
 
 def test_autotune_picks_a_working_configuration(gpu_available):

@@ -141,6 +141,8 @@ def test_pool_errors(gpu_available):
     d = L.DeviceBuffer(c.n_vars * 4)
     with pytest.raises(L.LdpcError):  # learned variant without a table
         p.decode_device(d.ptr, 4, d.ptr)
+    with pytest.raises(L.LdpcError):  # an empty call
+        L.PoolDecoder(c, L.MS, 10).decode_device(d.ptr, 0, d.ptr)
 
 
 def test_pool_simulate_matches_direct_decode(gpu_available):
