@@ -272,7 +272,8 @@ def run_plain(L, args, t_start):
                    "levels_per_iteration": cfg["n_levels"], "parallelism": "single GPU",
                    "pcie_inclusive": {"ms_per_decode": round(ms_pcie, 3),
                                       "info_gbps": round(frames * K / (ms_pcie * 1e-3) / 1e9, 3),
-                                      "buffers": "pinned host in/out, H2D + decode + D2H, synchronous",
+                                      "buffers": "pinned host in/out, H2D + decode + D2H in 8 overlapped chunks, "
+                                                 "synchronous call",
                                       "decisions_equal_device_path": same}},
         "roofline": roofline(alg, ms, traffic_for(args.traffic_json, f"oms_3_6_n4000_f{frames}_it{args.iters}")
                              if variant == L.OMS else None),

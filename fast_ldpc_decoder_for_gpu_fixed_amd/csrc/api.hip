@@ -80,6 +80,10 @@ struct ldpc_decoder {
     std::vector<int8_t> h_table;  // the learned table, for long_pool
     int early_term = 0;
     hipStream_t stream = nullptr;
+    // host-buffer decode in chunks: copy-in, second compute and copy-out
+    // streams, and the events that order the chunks (created on first use)
+    hipStream_t s_in = nullptr, s_c2 = nullptr, s_out = nullptr;
+    hipEvent_t ev_start = nullptr, ev_in[8] = {}, ev_dec[8] = {};
     // device
     uint2 *d_checks = nullptr;
     uint32_t *d_edge_off16 = nullptr;
@@ -121,6 +125,13 @@ void free_decoder(ldpc_decoder *d) {
     (void)hipFree(d->d_table);
     (void)hipFree(d->d_frames);
     if (d->long_pool) ldpc_pool_destroy(d->long_pool);
+    for (hipEvent_t e : d->ev_in)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : d->ev_dec)
+        if (e) (void)hipEventDestroy(e);
+    if (d->ev_start) (void)hipEventDestroy(d->ev_start);
+    for (hipStream_t s : {d->s_in, d->s_c2, d->s_out})
+        if (s) (void)hipStreamDestroy(s);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
@@ -511,10 +522,78 @@ int ldpc_decoder_decode_device(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d
     return launch(dec, a, s);
 }
 
+// Host-buffer decode of a large batch in K chunks of word blocks: chunk k's
+// H2D (copy-in stream), its decode (the decoder's stream and a second compute
+// stream in turn, so that one chunk's kernel tail overlaps the next one's
+// start) and its D2H (copy-out stream) are ordered by events, so the copies
+// of one chunk run under the decode of another.  Each chunk's frames are one
+// contiguous byte range of the frame-major buffers (OMS_v2's word-layout
+// input: a column range of every variable's row, one 2-D copy).
+constexpr int DECODE_CHUNKS = 8;
+static int decode_chunked(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations) {
+    if (!dec->s_in) {
+        for (hipStream_t *s : {&dec->s_in, &dec->s_c2, &dec->s_out}) HIP_TRY(hipStreamCreateWithFlags(s, hipStreamDefault));
+        HIP_TRY(hipEventCreateWithFlags(&dec->ev_start, hipEventDisableTiming));
+        for (int k = 0; k < DECODE_CHUNKS; k++) {
+            HIP_TRY(hipEventCreateWithFlags(&dec->ev_in[k], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&dec->ev_dec[k], hipEventDisableTiming));
+        }
+    }
+    if (dec->early_term) {
+        const int rc = reset_counters(dec, dec->stream);
+        if (rc) return rc;
+    }
+    // everything enqueued on the decoder's stream before this call comes first
+    HIP_TRY(hipEventRecord(dec->ev_start, dec->stream));
+    for (hipStream_t s : {dec->s_in, dec->s_c2, dec->s_out}) HIP_TRY(hipStreamWaitEvent(s, dec->ev_start, 0));
+    const size_t N = (size_t)dec->code.N, P = (size_t)dec->P, nblk = dec->W / P;
+    const bool words = dec->variant == LDPC_OMS_V2;
+    for (int k = 0; k < DECODE_CHUNKS; k++) {
+        const size_t b0 = nblk * k / DECODE_CHUNKS, b1 = nblk * (k + 1) / DECODE_CHUNKS;
+        const size_t f0 = b0 * P * 4 * N, fb = (b1 - b0) * P * 4 * N;  // frame-major byte range
+        if (words)
+            HIP_TRY(hipMemcpy2DAsync(dec->d_in + b0 * P * 4, dec->W * 4, in + b0 * P * 4, dec->W * 4, (b1 - b0) * P * 4,
+                                     N, hipMemcpyHostToDevice, dec->s_in));
+        else
+            HIP_TRY(hipMemcpyAsync(dec->d_in + f0, in + f0, fb, hipMemcpyHostToDevice, dec->s_in));
+        HIP_TRY(hipEventRecord(dec->ev_in[k], dec->s_in));
+        hipStream_t cs = (k & 1) ? dec->s_c2 : dec->stream;
+        HIP_TRY(hipStreamWaitEvent(cs, dec->ev_in[k], 0));
+        ldpc::DecodeArgs a = base_args(dec);
+        if (words) {
+            a.load_mode = ldpc::LOAD_WORDS;
+            a.in_words = reinterpret_cast<const uint32_t *>(dec->d_in);
+        } else {
+            a.load_mode = ldpc::LOAD_FRAMES;
+            a.in_frames = dec->d_in;
+        }
+        a.out_hard = dec->d_out;
+        a.iters = iterations;
+        a.first = 1;
+        if (!dec->early_term) { a.stopped = nullptr; a.iters_run = nullptr; }
+        a.block0 = b0;
+        a.nblocks = (unsigned)(b1 - b0);
+        const int rc = launch(dec, a, cs);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(dec->ev_dec[k], cs));
+        HIP_TRY(hipStreamWaitEvent(dec->s_out, dec->ev_dec[k], 0));
+        HIP_TRY(hipMemcpyAsync(out + f0, dec->d_out + f0, fb, hipMemcpyDeviceToHost, dec->s_out));
+    }
+    // the copy-out stream waited on every chunk's decode, which waited on its copy-in
+    HIP_TRY(hipStreamSynchronize(dec->s_out));
+    dec->loaded = false;
+    return LDPC_OK;
+}
+
 int ldpc_decoder_decode(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations) {
     if (!dec || !in || !out) return fail(LDPC_ERR_ARG, "null argument");
     int rc = check_iters(iterations);
     if (rc) return rc;
+    // chunks of at least 4 MiB and 512 workgroups each (the two compute
+    // streams fill each other's kernel tails)
+    if (!use_long_pool(dec) && io_bytes(dec) >= ((size_t)DECODE_CHUNKS << 22) &&
+        dec->W / dec->P >= (size_t)DECODE_CHUNKS * 512)
+        return decode_chunked(dec, in, out, iterations);
     HIP_TRY(hipMemcpyAsync(dec->d_in, in, io_bytes(dec), hipMemcpyHostToDevice, dec->stream));
     rc = ldpc_decoder_decode_device(dec, dec->d_in, dec->d_out, iterations, dec->stream);
     if (rc) return rc;

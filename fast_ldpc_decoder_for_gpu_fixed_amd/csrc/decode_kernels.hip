@@ -171,7 +171,7 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 16 ? 3 : 1))) void k_deco
     __shared__ int8_t s_lrow[MAX_DEG + 1];
 
     const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P;
-    const size_t b = blockIdx.x;
+    const size_t b = a.block0 + blockIdx.x;
     const int w = tid & (P - 1);
     uint32_t *post = POST_IN_LDS ? lds_post : a.post_state + b * (size_t)a.N * P;
 
@@ -334,7 +334,8 @@ static hipError_t launch_decode_t(const DecodeArgs &a, int threads, hipStream_t 
                                    (int)(MAX_LDS_BYTES - 1024));  // static flags live in the rest
     });
     if (e != hipSuccess) return e;
-    const unsigned nblk = (unsigned)(a.W / a.P);
+    if (a.block0 + a.nblocks > a.W / a.P) return hipErrorInvalidValue;
+    const unsigned nblk = a.nblocks ? a.nblocks : (unsigned)(a.W / a.P - a.block0);
     hipLaunchKernelGGL((k_decode<EQ, LDS, DMAXC, UD, G, VK>), dim3(nblk), dim3(threads), lds, s, a);
     return hipGetLastError();
 }

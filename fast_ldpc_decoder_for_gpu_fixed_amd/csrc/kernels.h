@@ -40,6 +40,8 @@ struct DecodeArgs {
     // blocking
     int P, log2P;              // words per workgroup (power of two)
     size_t W;                  // words in the batch (multiple of P)
+    size_t block0;             // the launch covers word blocks block0 ..
+    unsigned nblocks;          //   block0 + nblocks - 1 (nblocks 0: all W/P)
     // state
     uint32_t *post_state;      // [W/P][N][P] words (internal posterior)
     uint32_t *msgs;            // [W/P][E][P] words, per check [word][j]

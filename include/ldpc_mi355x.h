@@ -132,7 +132,10 @@ void ldpc_decoder_destroy(ldpc_decoder *dec);
 /* decode(var_nodes, Rprime_fix, nombre_iterations): synchronous host-buffer
  * decode, H2D -> interleave -> iterations -> hard decision -> D2H
  * (src/decoder_oms/CGPU_Decoder_OMS_SIMD.cu:97-149).
- * in: 4W*N int8 (frame-major; var-major words for OMS_v2); out: 4W*N bytes. */
+ * in: 4W*N int8 (frame-major; var-major words for OMS_v2); out: 4W*N bytes.
+ * A batch of 32 MiB and 4096 word blocks or more is copied and decoded in 8
+ * chunks whose copies overlap the other chunks' decodes (pinned host buffers
+ * make the copies asynchronous); the result is the same. */
 int ldpc_decoder_decode(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations);
 /* Same on device buffers, enqueued on `stream` (NULL = the decoder's own).
  * Asynchronous: returns after the launch.  Frames whose 4-frame word block

@@ -244,6 +244,29 @@ def test_full_size_batch_sampled(variant, gpu_available):
     np.testing.assert_array_equal(out.reshape(4 * W, N)[frames].ravel(), orc.hard())
 
 
+@pytest.mark.parametrize("variant,early", [(L.OMS, 0), (L.OMS_V2, 0), (L.NMS2, 1), (L.LOMS, 0)],
+                         ids=["OMS", "OMS_v2", "2NMS_early", "LOMS"])
+def test_host_decode_in_chunks_equals_device_path(variant, early, gpu_available):
+    """2^16 frames through decode() from host buffers, which copies and
+    decodes them in overlapped chunks of word blocks (OMS_v2: column ranges
+    of its word-layout input), against the one-launch device path."""
+    code = code36()
+    W, N = 16384, 4000
+    rng = np.random.default_rng(variant)
+    x = rng.integers(-40, 41, size=4 * W * N, dtype=np.int8)
+    x[: 4 * N * 64] = channel_llr(N, 4 * 64, seed=3).reshape(-1)  # some decodable frames as well
+    dec = L.make_decoder(code, W, variant, early_term=early)
+    assert dec.config()["words_per_block"] * 8 * 512 <= W  # the chunked path
+    if variant == L.LOMS:
+        dec.set_table(learned_table(variant))
+    host = dec.decode_bytes(x, 12)
+    d_in, d_out = L.DeviceBuffer(x.nbytes), L.DeviceBuffer(x.nbytes)
+    d_in.upload(x)
+    dec.decode_device(d_in.ptr, d_out.ptr, 12)
+    L.synchronize()
+    np.testing.assert_array_equal(host, d_out.download(np.empty(x.nbytes, np.uint8)))
+
+
 def test_device_interleave_roundtrip(gpu_available):
     N, W = 4000, 96
     llr = channel_llr(N, 4 * W, seed=5)
