@@ -530,15 +530,18 @@ int ldpc_decoder_decode_device(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d
 // contiguous byte range of the frame-major buffers (OMS_v2's word-layout
 // input: a column range of every variable's row, one 2-D copy).
 constexpr int DECODE_CHUNKS = 8;
-static int decode_chunked(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations) {
-    if (!dec->s_in) {
-        for (hipStream_t *s : {&dec->s_in, &dec->s_c2, &dec->s_out}) HIP_TRY(hipStreamCreateWithFlags(s, hipStreamDefault));
-        HIP_TRY(hipEventCreateWithFlags(&dec->ev_start, hipEventDisableTiming));
-        for (int k = 0; k < DECODE_CHUNKS; k++) {
-            HIP_TRY(hipEventCreateWithFlags(&dec->ev_in[k], hipEventDisableTiming));
-            HIP_TRY(hipEventCreateWithFlags(&dec->ev_dec[k], hipEventDisableTiming));
-        }
+static int chunk_streams(ldpc_decoder *dec) {  // creates what is missing (idempotent)
+    for (hipStream_t *s : {&dec->s_in, &dec->s_c2, &dec->s_out})
+        if (!*s) HIP_TRY(hipStreamCreateWithFlags(s, hipStreamDefault));
+    for (hipEvent_t *e : {&dec->ev_start})
+        if (!*e) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    for (int k = 0; k < DECODE_CHUNKS; k++) {
+        if (!dec->ev_in[k]) HIP_TRY(hipEventCreateWithFlags(&dec->ev_in[k], hipEventDisableTiming));
+        if (!dec->ev_dec[k]) HIP_TRY(hipEventCreateWithFlags(&dec->ev_dec[k], hipEventDisableTiming));
     }
+    return LDPC_OK;
+}
+static int enqueue_chunks(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations) {
     if (dec->early_term) {
         const int rc = reset_counters(dec, dec->stream);
         if (rc) return rc;
@@ -579,10 +582,22 @@ static int decode_chunked(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int
         HIP_TRY(hipStreamWaitEvent(dec->s_out, dec->ev_dec[k], 0));
         HIP_TRY(hipMemcpyAsync(out + f0, dec->d_out + f0, fb, hipMemcpyDeviceToHost, dec->s_out));
     }
-    // the copy-out stream waited on every chunk's decode, which waited on its copy-in
-    HIP_TRY(hipStreamSynchronize(dec->s_out));
-    dec->loaded = false;
     return LDPC_OK;
+}
+static int decode_chunked(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations) {
+    int rc = chunk_streams(dec);
+    if (rc) return rc;
+    rc = enqueue_chunks(dec, in, out, iterations);
+    // the copy-out stream waited on every chunk's decode, which waited on its
+    // copy-in; after an error, whatever was enqueued still ends before the
+    // call returns (the caller's buffers may be freed right after)
+    if (rc == LDPC_OK) {
+        HIP_TRY(hipStreamSynchronize(dec->s_out));
+    } else {
+        for (hipStream_t s : {dec->s_in, dec->stream, dec->s_c2, dec->s_out}) (void)hipStreamSynchronize(s);
+    }
+    dec->loaded = false;
+    return rc;
 }
 
 int ldpc_decoder_decode(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int iterations) {
@@ -596,7 +611,10 @@ int ldpc_decoder_decode(ldpc_decoder *dec, const int8_t *in, uint8_t *out, int i
         return decode_chunked(dec, in, out, iterations);
     HIP_TRY(hipMemcpyAsync(dec->d_in, in, io_bytes(dec), hipMemcpyHostToDevice, dec->stream));
     rc = ldpc_decoder_decode_device(dec, dec->d_in, dec->d_out, iterations, dec->stream);
-    if (rc) return rc;
+    if (rc) {
+        (void)hipStreamSynchronize(dec->stream);  // the copy-in reads `in`: it ends before the call returns
+        return rc;
+    }
     HIP_TRY(hipMemcpyAsync(out, dec->d_out, io_bytes(dec), hipMemcpyDeviceToHost, dec->stream));
     HIP_TRY(hipStreamSynchronize(dec->stream));
     return LDPC_OK;
