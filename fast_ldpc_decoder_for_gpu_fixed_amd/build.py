@@ -6,7 +6,8 @@
 The product is ``libldpc_mi355x.so`` next to this file: HIP kernels for gfx950
 plus the C-ABI of ``include/ldpc_mi355x.h``.  Objects are compiled in
 parallel with ``hipcc`` and linked in-tree so the library travels with the
-source tree.
+source tree.  ``ldpc_sim`` beside it is the Eb/N0 sweep driver (the
+reference's ``src/main.cpp``), a host program on that C-ABI.
 """
 from __future__ import annotations
 
@@ -21,6 +22,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libldpc_mi355x.so"
+SIM = PKG / "ldpc_sim"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -Xclang -target-feature -Xclang -packed-fp32-ops keeps hipcc from forming
@@ -57,7 +59,17 @@ def build_lib() -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
     os.replace(tmp, LIB)
+    build_sim()
     return LIB
+
+
+def build_sim() -> Path:
+    """The sweep driver: plain C++ on the C-ABI, linked to the in-tree library."""
+    tmp = SIM.with_suffix(".tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Werror", f"-I{ROOT / 'include'}", str(CSRC / "ldpc_sim.cpp"),
+          f"-L{PKG}", "-lldpc_mi355x", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
+    os.replace(tmp, SIM)
+    return SIM
 
 
 def build_oracle() -> Path:

@@ -81,7 +81,7 @@ ldpc_code *ldpc_code_create_grouped(int n_vars, int deg1, int count1, int deg2,
 ldpc_code *ldpc_code_load_table(const char *path, int n_vars);
 /* Seeded generators (deterministic across platforms).
  * Regular (dv, dc): random socket permutation, 4-cycles and repeated
- * variables rejected (BASELINE configs 1/2/4). */
+ * variables rejected (the (3,6) N = 4000 code: n_vars 4000, dv 3, dc 6). */
 ldpc_code *ldpc_code_regular(int n_vars, int dv, int dc, uint64_t seed);
 /* Quasi-cyclic: mb x nb base graph, circulant size z, dual-diagonal parity
  * part on the last mb columns, info entries present with probability p (an

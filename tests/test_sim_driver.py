@@ -1,0 +1,81 @@
+"""The Eb/N0 sweep driver (ldpc_sim, the reference's src/main.cpp on the C
+ABI).  CPU: its command line (unknown options, -info, no device).  GPU: a
+point that ends after one batch prints the frame and bit errors that the
+library's one-point simulation gives on the same seed, for the drop-in and
+the frame-queue decoder; a sweep stops each point at the frame-error limit,
+its FER falls with the SNR, and the quasi-error-free limit ends the sweep."""
+import re
+import subprocess
+
+import pytest
+
+import fast_ldpc_decoder_for_gpu_fixed_amd as L
+from fast_ldpc_decoder_for_gpu_fixed_amd import build
+
+GOLDEN = 0x9E3779B97F4A7C15  # the driver's per-batch seed step
+POINT = re.compile(r"SNR = ([-\d.]+) \| BER =\s+(\S+) \| FER =\s+(\S+) \|.*MATRICES =\s+(\d+)\| FE = (\d+) \| "
+                   r"BE = (\d+) \|.*AVG IT = ([\d.]+)")
+
+
+def sim():
+    if not build.SIM.exists():
+        build.build_sim()
+    return str(build.SIM)
+
+
+def run(*args, timeout=300):
+    return subprocess.run([sim(), *args], capture_output=True, text=True, timeout=timeout)
+
+
+def points(out):
+    return [(float(m[1]), float(m[3]), int(m[4]), int(m[5]), int(m[6]), float(m[7])) for m in POINT.finditer(out)]
+
+
+def test_unknown_option_is_refused_with_usage():
+    r = run("-bogus")
+    assert r.returncode == 2 and "Unknown argument" in r.stdout and "usage" in r.stdout
+
+
+def test_info_names_the_library():
+    r = run("-info")
+    assert r.returncode == 0 and "ldpc_mi355x" in r.stdout
+
+
+def test_no_device_is_an_error_here():
+    if L.device_count() > 0:
+        pytest.skip("a device is present")
+    r = run("-OMS", "-min", "1", "-max", "1")
+    assert r.returncode == 1 and "no gfx950 device" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("pool", [False, True], ids=["drop_in", "frame_queue"])
+def test_one_batch_point_equals_library_simulation(pool, gpu_available):
+    W = 256
+    args = ["-OMS", "-min", "1.0", "-max", "1.0", "-fer", "1", "-n", str(W), "-iter", "12", "-seed", "7"]
+    r = run(*(args + (["-pool"] if pool else [])))
+    assert r.returncode == 0, r.stdout + r.stderr
+    (snr, fer, frames, fe, be, _), = points(r.stdout)
+    assert frames == 4 * W and fe >= 1  # 1.0 dB: the first batch already has frame errors
+    code = L.code_3_6(4000, seed=1)
+    seed = (7 + GOLDEN) % (1 << 64)
+    if pool:
+        ref = L.PoolDecoder(code, L.OMS, 12).simulate(1.0, 4 * W, seed=seed, snr_idx=0)
+    else:
+        ref = L.make_decoder(code, W, L.OMS).simulate(1.0, 4 * W, 12, seed=seed, snr_idx=0)
+    assert (fe, be) == (ref["frame_errors"], ref["bit_errors"])
+
+
+@pytest.mark.gpu
+def test_sweep_stops_at_the_frame_error_limit_and_at_qef(gpu_available):
+    r = run("-OMS", "-min", "1.0", "-max", "2.0", "-pas", "0.5", "-fer", "40", "-n", "1024", "-iter", "20")
+    assert r.returncode == 0, r.stdout + r.stderr
+    pts = points(r.stdout)
+    assert [p[0] for p in pts] == [1.0, 1.5, 2.0]
+    assert all(p[3] >= 40 for p in pts)  # every point reached the limit
+    assert pts[0][1] > pts[1][1] > pts[2][1]  # the waterfall
+    # quasi error free: a BER limit above the first point's BER ends the sweep there
+    q = run("-OMS", "-min", "2.5", "-max", "3.5", "-pas", "0.5", "-fer", "1", "-n", "1024", "-qef", "1e-2",
+            "-maxframes", "65536")
+    assert q.returncode == 0, q.stdout + q.stderr
+    assert len(points(q.stdout)) == 1 and "QUASI-ERROR FREE" in q.stdout
