@@ -54,8 +54,8 @@ def train(code: Code, iters: int, batch: int, steps: int, lr: float, ebn0, seed:
 
 
 def evaluate(code: Code, table: np.ndarray, variant: int, iters: int, ebn0_list, frames: int, seed: int,
-             words: int = 16384, snr_base: int = 100):
-    """Decode identical frames with the plain and learned decoders."""
+             words: int = 16384, snr_base: int = 100, quant: int = 0):
+    """Decode identical frames (LLR quantizer `quant`) with the plain and learned decoders."""
     K = code.n_info
     out = []
     decs = {
@@ -74,9 +74,10 @@ def evaluate(code: Code, table: np.ndarray, variant: int, iters: int, ebn0_list,
     }
     for i, eb in enumerate(ebn0_list):
         row = {"ebn0_db": eb}
-        runs = [(name, lambda d=dec, it=it: d.simulate(eb, frames, it, seed=seed, snr_idx=snr_base + i))
+        runs = [(name, lambda d=dec, it=it: d.simulate(eb, frames, it, seed=seed, snr_idx=snr_base + i,
+                                                       quant=quant))
                 for name, (dec, it) in decs.items()]
-        runs += [(name, lambda p=pool: p.simulate(eb, frames, seed=seed, snr_idx=snr_base + i))
+        runs += [(name, lambda p=pool: p.simulate(eb, frames, seed=seed, snr_idx=snr_base + i, quant=quant))
                  for name, pool in pools.items()]
         for name, run in runs:
             r = run()
@@ -116,6 +117,12 @@ def load_table(path: str):
     return (LOMS if rec["variant"] == "LOMS" else LNMS), np.asarray(rec["table"], dtype=np.int8)
 
 
+def table_quantizer(path: str):
+    """The LLR quantizer a saved table was trained with (None if not recorded)."""
+    with open(path) as f:
+        return json.load(f).get("training", {}).get("quantizer")
+
+
 def ira_codes():
     """(training code N=6480, evaluation code N=64800): info degrees 8 (40%) and
     3 (60%), staircase parity, colour-ordered tables."""
@@ -138,6 +145,9 @@ def main(argv=None):
     ap.add_argument("--eval-seed", type=int, default=777)
     ap.add_argument("--save-table", default="", help="write the quantised table and its training record (JSON)")
     ap.add_argument("--no-eval", action="store_true")
+    ap.add_argument("--quant", type=int, choices=[0, 1], default=None,
+                    help="LLR quantizer: 0 sat127(round(16y/sigma^2)), 1 trunc(8y) clamped to +-31 "
+                         "(default: 0 for the (3,6) code, 1 for IRA; with --eval-table, the table's own)")
     ap.add_argument("--eval-table", default="",
                     help="evaluate a table written with --save-table instead of training one")
     ap.add_argument("--drop-last-row", type=int, default=None,
@@ -153,7 +163,7 @@ def main(argv=None):
     hi = a.ebn0_hi if a.ebn0_hi is not None else (1.6 if ira else 2.5)
     eval_ebn0 = a.eval_ebn0 or ([1.0, 1.15, 1.3, 1.45, 1.6] if ira else [1.5, 1.75, 2.0, 2.25, 2.5])
     eval_frames = a.eval_frames or ((1 << 16) if ira else (1 << 22))
-    quant = 1 if ira else 0
+    quant = a.quant if a.quant is not None else (1 if ira else 0)
 
     if ira:
         code, eval_code = ira_codes()
@@ -164,10 +174,14 @@ def main(argv=None):
     col = code.max_deg
     if a.eval_table:
         variant, table = load_table(a.eval_table)
+        if a.quant is None and table_quantizer(a.eval_table) is not None:
+            quant = table_quantizer(a.eval_table)
         iters = table.shape[0]
         res = {"code": desc, "table": a.eval_table, "int8_table_deg%d" % col: [int(x) for x in table[:, col]]}
+        res["quantizer"] = quant
         res["evaluation"] = (evaluate_ira(eval_code, table, variant, 50, eval_ebn0, eval_frames, a.eval_seed) if ira
-                             else evaluate(code, table, variant, iters, eval_ebn0, eval_frames, a.eval_seed))
+                             else evaluate(code, table, variant, iters, eval_ebn0, eval_frames, a.eval_seed,
+                                           quant=quant))
         json.dump(res, sys.stdout, indent=1)
         print()
         return
@@ -198,7 +212,8 @@ def main(argv=None):
         if ira:
             res["evaluation"] = evaluate_ira(eval_code, table, variant, 50, eval_ebn0, eval_frames, a.eval_seed)
         else:
-            res["evaluation"] = evaluate(code, table, variant, iters, eval_ebn0, eval_frames, a.eval_seed)
+            res["evaluation"] = evaluate(code, table, variant, iters, eval_ebn0, eval_frames, a.eval_seed,
+                                         quant=quant)
     json.dump(res, sys.stdout, indent=1)
     print()
 
