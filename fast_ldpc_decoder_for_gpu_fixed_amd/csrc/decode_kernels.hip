@@ -267,13 +267,15 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 16 ? 3 : 1))) void k_deco
 // device interleave (src/transpose/GPU_Transpose_uint8.cu:80-130) and inverse
 // with hard decision (:9-78), as LDS-tiled transposes: a workgroup moves a
 // tile of TV variables x TW words; global reads and writes are contiguous.
+// Word tiles run along grid x (up to 2^31 - 1 of them), variable tiles
+// along grid y (N up to 65535 * TV).
 // ---------------------------------------------------------------------------
 constexpr int TV = 64, TW = 64;
 
 __global__ __launch_bounds__(256) void k_interleave(const int8_t *__restrict__ in,
                                                     uint32_t *__restrict__ out, int N, int W) {
     __shared__ uint32_t tile[TV][TW + 1];
-    const int v0 = blockIdx.x * TV, w0 = blockIdx.y * TW, tid = threadIdx.x;
+    const int v0 = blockIdx.y * TV, w0 = blockIdx.x * TW, tid = threadIdx.x;
     uint8_t *tb = reinterpret_cast<uint8_t *>(&tile[0][0]);
     const int rowb = (TW + 1) * 4;
     // read 4*TW frames x TV bytes
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(256) void k_interleave(const int8_t *__restrict__ i
         const int fl = i / TV, vl = i % TV;
         const int v = v0 + vl, wl = fl >> 2;
         if (v < N && w0 + wl < W)
-            tb[vl * rowb + fl] = (uint8_t)in[((size_t)(w0 * 4 + fl)) * N + v];
+            tb[vl * rowb + fl] = (uint8_t)in[((size_t)w0 * 4 + fl) * N + v];
     }
     __syncthreads();
     for (int i = tid; i < TV * TW; i += 256) {
@@ -296,7 +298,7 @@ template <bool HD>
 __global__ __launch_bounds__(256) void k_deinterleave(const uint32_t *__restrict__ in,
                                                       uint8_t *__restrict__ out, int N, int W) {
     __shared__ uint32_t tile[TV][TW + 1];
-    const int v0 = blockIdx.x * TV, w0 = blockIdx.y * TW, tid = threadIdx.x;
+    const int v0 = blockIdx.y * TV, w0 = blockIdx.x * TW, tid = threadIdx.x;
     for (int i = tid; i < TV * TW; i += 256) {
         const int vl = i / TW, wl = i % TW;
         uint32_t x = 0;
@@ -316,7 +318,7 @@ __global__ __launch_bounds__(256) void k_deinterleave(const uint32_t *__restrict
     for (int i = tid; i < 4 * TW * TV; i += 256) {
         const int fl = i / TV, vl = i % TV;
         const int v = v0 + vl, wl = fl >> 2;
-        if (v < N && w0 + wl < W) out[((size_t)(w0 * 4 + fl)) * N + v] = tb[vl * rowb + fl];
+        if (v < N && w0 + wl < W) out[((size_t)w0 * 4 + fl) * N + v] = tb[vl * rowb + fl];
     }
 }
 
@@ -397,19 +399,19 @@ hipError_t launch_decode(const DecodeArgs &a, int dmax, int threads, bool post_i
 }
 
 hipError_t launch_interleave(const int8_t *in, uint32_t *out, int N, int W, hipStream_t s) {
-    dim3 g((unsigned)((N + TV - 1) / TV), (unsigned)((W + TW - 1) / TW));
+    dim3 g((unsigned)((W + TW - 1) / TW), (unsigned)((N + TV - 1) / TV));
     hipLaunchKernelGGL(k_interleave, g, dim3(256), 0, s, in, out, N, W);
     return hipGetLastError();
 }
 
 hipError_t launch_deinterleave_hd(const uint32_t *in, uint8_t *out, int N, int W, hipStream_t s) {
-    dim3 g((unsigned)((N + TV - 1) / TV), (unsigned)((W + TW - 1) / TW));
+    dim3 g((unsigned)((W + TW - 1) / TW), (unsigned)((N + TV - 1) / TV));
     hipLaunchKernelGGL(k_deinterleave<true>, g, dim3(256), 0, s, in, out, N, W);
     return hipGetLastError();
 }
 
 hipError_t launch_deinterleave_raw(const uint32_t *in, int8_t *out, int N, int W, hipStream_t s) {
-    dim3 g((unsigned)((N + TV - 1) / TV), (unsigned)((W + TW - 1) / TW));
+    dim3 g((unsigned)((W + TW - 1) / TW), (unsigned)((N + TV - 1) / TV));
     hipLaunchKernelGGL(k_deinterleave<false>, g, dim3(256), 0, s, in, reinterpret_cast<uint8_t *>(out), N, W);
     return hipGetLastError();
 }
