@@ -272,6 +272,29 @@ def test_noiseless_decodes_to_zero_and_bounds(variant):
         assert O.syndrome_ok(dec.hard(), 480, deg, pos).all()
 
 
+def test_nms_saturation_trajectory_on_noiseless_frames():
+    """Known answer for the unclamped NMS on an all-(-127) frame, derived by
+    hand from src/decoder_nms/cuda/CUDA_NMS_SIMD.cu:40-132 and :160-220.
+    Iteration 1 reads no messages; every |t| is 127 (or |-128| = 128 for a
+    variable an earlier check already moved, above the 127 the minima start
+    from), so every message is -trunc(0.75*127) = -95 and every posterior
+    saturates at -128.  In iteration 2 the first check of the table sees
+    t = -128 - (-95) = -33 on all its edges and sends -trunc(0.75*33) = -24:
+    the magnitude lost to saturation is what makes the decoder oscillate."""
+    import fast_ldpc_decoder_for_gpu_fixed_amd as L
+    code = L.code_3_6(480, seed=2)
+    deg, pos = code.export()
+    W = 2
+    dec = O.OracleDecoder(deg, pos, 480, W, O.NMS, early_term=0)
+    dec.load(np.full(4 * W * 480, -127, np.int8))
+    dec.iterate(1)
+    assert (dec.post.view(np.int8) == -128).all()
+    assert (dec.msg.view(np.int8) == -95).all()
+    dec.iterate(1)
+    first = dec.msg.view(np.int8).reshape(-1, 4 * W)[: int(deg[0])]  # [edge][frame] of check 0
+    assert (first == -24).all()
+
+
 def test_early_term_stops_on_zero_parity():
     import fast_ldpc_decoder_for_gpu_fixed_amd as L
     code = L.code_3_6(480, seed=2)
