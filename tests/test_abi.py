@@ -111,3 +111,97 @@ def test_cpp_classes_run_on_the_gpu(tmp_path, gpu_available):
     import subprocess
     run = subprocess.run([str(build_cpp_user(tmp_path))], capture_output=True, text=True)
     assert run.returncode == 0 and "ok" in run.stdout, run.stdout + run.stderr
+
+
+CPP_DECODE = r'''
+#include "CGPUDecoder.hpp"
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+// The reference's calling sequence (src/main.cpp:212-229, 267): one decoder
+// object per variant, CTrame's pinned buffers, decode(t_noise_data,
+// t_decode_data, iterations).  argv: directory, words W, iterations.
+static std::vector<char> slurp(const std::string &p, size_t n) {
+    std::vector<char> b(n);
+    FILE *f = std::fopen(p.c_str(), "rb");
+    if (!f || std::fread(b.data(), 1, n, f) != n) { std::fprintf(stderr, "read %s\n", p.c_str()); std::exit(3); }
+    std::fclose(f);
+    return b;
+}
+int main(int argc, char **argv) {
+    if (argc != 4) return 1;
+    const std::string dir = argv[1];
+    const size_t W = std::stoul(argv[2]);
+    const int iters = std::atoi(argv[3]);
+    ldpc_code *base = ldpc_code_regular(4000, 3, 6, 1);
+    ldpc_code *code = base ? ldpc_code_reorder_by_colour(base) : nullptr;
+    if (!code) return 2;
+    int n, k, m;
+    ldpc_code_info(code, &n, &k, &m, nullptr, nullptr, nullptr);
+    const size_t bytes = 4 * W * (size_t)n;
+    const std::vector<char> frames = slurp(dir + "/in_frames.bin", bytes);
+    const std::vector<char> words = slurp(dir + "/in_words.bin", bytes);
+    CTrame trame(n, k, (int)W);
+    const char *names[5] = {"MS", "OMS", "NMS", "2NMS", "OMS_v2"};
+    for (int v = 0; v < 5; v++) {
+        std::unique_ptr<CGPUDecoder> d;
+        switch (v) {
+        case 0: d.reset(new CGPU_Decoder_MS_SIMD(W, n, k, m, code)); break;
+        case 1: d.reset(new CGPU_Decoder_OMS_SIMD(W, n, k, m, code)); break;
+        case 2: d.reset(new CGPU_Decoder_NMS_SIMD(W, n, k, m, code)); break;
+        case 3: d.reset(new CGPU_Decoder_2NMS_SIMD(W, n, k, m, code)); break;
+        default: d.reset(new CGPU_Decoder_MS_SIMD_v2(W, n, k, m, code)); break;
+        }
+        d->initialize();
+        // the xMS decoder takes the device word layout (src/decoder_oms_v2/CGPU_Decoder_MS_SIMD_v2.cu:187)
+        std::memcpy(trame.get_t_noise_data(), (v == 4 ? words : frames).data(), bytes);
+        std::memset(trame.get_t_decode_data(), 0xAA, bytes);
+        d->decode(trame.get_t_noise_data(), trame.get_t_decode_data(), iters);
+        const std::string out = dir + "/out_" + names[v] + ".bin";
+        FILE *f = std::fopen(out.c_str(), "wb");
+        if (!f || std::fwrite(trame.get_t_decode_data(), 1, bytes, f) != bytes) return 4;
+        std::fclose(f);
+    }
+    ldpc_code_destroy(code);
+    ldpc_code_destroy(base);
+    std::printf("decoded\n");
+    return 0;
+}
+'''
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_decodes_every_variant(tmp_path, gpu_available):
+    """A C++ caller drives the five decoder subclasses through
+    include/CGPUDecoder.hpp exactly as the reference's main.cpp does (CTrame
+    pinned buffers, decode(float*, int*, iterations); the 2NMS object with its
+    default early stop), and each output equals the oracle's decisions."""
+    import subprocess
+    from oracle import oracle as O
+    root = HEADER.parents[1]
+    pkg = root / "fast_ldpc_decoder_for_gpu_fixed_amd"
+    src, exe = tmp_path / "dropin.cpp", tmp_path / "dropin"
+    src.write_text(CPP_DECODE)
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", f"-I{root / 'include'}", str(src), "-o",
+                        str(exe), f"-L{pkg}", "-lldpc_mi355x", f"-Wl,-rpath,{pkg}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    W, iters, N = 64, 20, 4000
+    rng = np.random.default_rng(17)
+    sigma = np.sqrt(1.0 / (2 * 0.5 * 10 ** 0.2))
+    llr = np.clip(np.round(16.0 * (-1.0 + sigma * rng.standard_normal(4 * W * N)) / sigma ** 2),
+                  -127, 127).astype(np.int8)
+    (tmp_path / "in_frames.bin").write_bytes(llr.tobytes())
+    (tmp_path / "in_words.bin").write_bytes(O.interleave(llr, N).tobytes())
+    run = subprocess.run([str(exe), str(tmp_path), str(W), str(iters)], capture_output=True, text=True, timeout=300)
+    assert run.returncode == 0 and "decoded" in run.stdout, run.stdout + run.stderr
+    code = L.code_3_6(N, seed=1)
+    deg, pos = code.export()
+    for name, variant, early in (("MS", L.MS, 0), ("OMS", L.OMS, 0), ("NMS", L.NMS, 0), ("2NMS", L.NMS2, 1),
+                                 ("OMS_v2", L.OMS_V2, 0)):
+        orc = O.OracleDecoder(deg, pos, N, W, variant, q1_split=code.q1_split, early_term=early)
+        orc.load(llr)
+        orc.iterate(iters, threads=8)
+        got = np.frombuffer((tmp_path / f"out_{name}.bin").read_bytes(), np.uint8)
+        np.testing.assert_array_equal(got, orc.hard(), err_msg=name)

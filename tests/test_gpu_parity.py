@@ -2,9 +2,9 @@
 
 Bar: bit-exact posterior and message arrays after EVERY iteration, every
 variant, plus hard decisions.  Sizes are chosen so the oracle finishes in
-seconds; the full-size batch (2^16 frames) is checked on a sample of words
-(the oracle decodes words independently, so a sample is an exact check of
-those words)."""
+seconds; the full-size batch (2^16 frames) is compared whole: every decision
+at 10 and 20 iterations for the five variants, and the complete posterior and
+message arrays for OMS."""
 import os
 
 import numpy as np
@@ -221,27 +221,59 @@ def test_global_posterior_path(gpu_available):
         run_lockstep(qc, W, variant, llr, 5, post_in_lds=0, words_per_block=4)
 
 
+def full_batch_llr():
+    """The drop-in's full-size batch: 2^16 frames (W = 16384) of the (3,6)
+    N=4000 code at 2.0 dB."""
+    if "full" not in _CODES:
+        _CODES["full"] = channel_llr(4000, 4 * 16384, ebn0=2.0, seed=2024)
+    return _CODES["full"]
+
+
 @pytest.mark.parametrize("variant", [L.OMS, L.MS, L.NMS, L.NMS2, L.OMS_V2], ids=lambda v: L.VARIANT_NAMES[v])
-def test_full_size_batch_sampled(variant, gpu_available):
-    """2^16 frames (W = 16384) at 2.0 dB, 20 iterations, no early stop:
-    the GPU's hard decisions for 96 sampled words equal the oracle's."""
+def test_full_size_batch(variant, gpu_available):
+    """2^16 frames at 2.0 dB, one decode() of the whole batch at 10 and at 20
+    fixed iterations, no early stop (src/decoder_oms/CGPU_Decoder_OMS_SIMD.cu:97-149):
+    every decision of every frame equals the oracle's."""
     code = code36()
     deg, pos = code.export()
-    W = 16384
-    N = 4000
-    if "full" not in _CODES:
-        _CODES["full"] = channel_llr(N, 4 * W, ebn0=2.0, seed=2024)
-    llr = _CODES["full"]
-    dec = L.make_decoder(code, W, variant, early_term=0)
+    W, N = 16384, 4000
+    llr = full_batch_llr()
     x = O.interleave(llr, N).view(np.int8) if variant == L.OMS_V2 else llr
-    out = dec.decode_bytes(x, 20)
-    words = np.unique(np.concatenate([np.arange(0, W, 173), [W - 1, W - 2, 1]]))[:96]
-    frames = (4 * words[:, None] + np.arange(4)[None, :]).ravel()
-    sub = llr.reshape(4 * W, N)[frames].ravel()
-    orc = O.OracleDecoder(deg, pos, N, len(words), variant, q1_split=code.q1_split, early_term=0)
-    orc.load(sub)
+    orc = O.OracleDecoder(deg, pos, N, W, variant, q1_split=code.q1_split, early_term=0)
+    orc.load(llr)
+    done = 0
+    for iters in (10, 20):
+        print(f"full batch {L.VARIANT_NAMES[variant]} W={W} iterations={iters}", flush=True)
+        dec = L.make_decoder(code, W, variant, early_term=0)
+        out = dec.decode_bytes(x, iters)
+        del dec
+        orc.iterate(iters - done, threads=THREADS)
+        done = iters
+        ref = orc.hard()
+        bad = np.nonzero((out != ref).reshape(4 * W, N).any(axis=1))[0]
+        assert bad.size == 0, f"{bad.size} frames differ after {iters} iterations, first {bad[:8].tolist()}"
+
+
+def test_full_size_batch_state(gpu_available):
+    """OMS, 2^16 frames, 20 iterations through the state API: the whole
+    posterior [v][W] and message [e][W] arrays equal the oracle's
+    (the reference's device_V / d_MSG_C_2_V, src/decoder_template/CGPUDecoder.cpp:14-38)."""
+    code = code36()
+    deg, pos = code.export()
+    W, N = 16384, 4000
+    llr = full_batch_llr()
+    dec = L.make_decoder(code, W, L.OMS, early_term=0)
+    dec.load(llr)
+    dec.iterate(20)
+    post, msg = dec.get_state()
+    hard = dec.get_hard()
+    del dec
+    orc = O.OracleDecoder(deg, pos, N, W, L.OMS, q1_split=code.q1_split, early_term=0)
+    orc.load(llr)
     orc.iterate(20, threads=THREADS)
-    np.testing.assert_array_equal(out.reshape(4 * W, N)[frames].ravel(), orc.hard())
+    np.testing.assert_array_equal(post, orc.post, err_msg="posterior after 20 iterations")
+    np.testing.assert_array_equal(msg, orc.msg, err_msg="messages after 20 iterations")
+    np.testing.assert_array_equal(hard, orc.hard())
 
 
 @pytest.mark.parametrize("variant,early", [(L.OMS, 0), (L.OMS_V2, 0), (L.NMS2, 1), (L.LOMS, 0)],
