@@ -485,6 +485,17 @@ class MixedDecoder:
         check(load().ldpc_mixed_decode(self._h, v(d_in), v(d_offsets), v(d_code_ids), frames, v(d_out), v(d_iters),
                                        int(graph), v(stream)), "mixed_decode")
 
+    def decode_device_noalloc(self, d_in: int, d_offsets: int, d_code_ids: int, frames: int, d_out: int,
+                              d_iters: int = 0, stream: int = 0) -> None:
+        """The same step with its scratch in the library's static device array
+        (no allocation; frames * codes <= NOALLOC_ENTRIES; calls overlapping on
+        one GPU race)."""
+        v = lambda p: C.c_void_p(p) if p else None
+        check(load().ldpc_mixed_decode_noalloc(self._h, v(d_in), v(d_offsets), v(d_code_ids), frames, v(d_out),
+                                               v(d_iters), v(stream)), "mixed_decode_noalloc")
+
+    NOALLOC_ENTRIES = 1 << 21  # LDPC_MIXED_NOALLOC_ENTRIES
+
     def config(self, code_index: int) -> dict:
         P, T, G, U, B, Q = (C.c_int() for _ in range(6))
         check(load().ldpc_mixed_config(self._h, code_index, C.byref(P), C.byref(T), C.byref(G), C.byref(U),

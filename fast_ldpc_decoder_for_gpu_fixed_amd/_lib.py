@@ -92,6 +92,8 @@ SIGNATURES = {
     "ldpc_mixed_config": (C.c_int, [C.c_void_p, C.c_int, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p,
                                     _c_int_p]),
     "ldpc_mixed_destroy": (None, [C.c_void_p]),
+    "ldpc_mixed_decode_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]),
     "ldpc_pool_config": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "ldpc_pool_stream": (C.c_void_p, [C.c_void_p]),
     "ldpc_pool_destroy": (None, [C.c_void_p]),

@@ -10,10 +10,15 @@
 // workgroups free when its queue runs dry.  The step is kernels and event
 // edges only (queue counters are cleared by kernels), so it is captured once
 // into a hipGraph and replayed while the buffers stay the same.
+//
+// ldpc_mixed_decode_noalloc runs the same step with its scratch (the per-code
+// queue counters and frame lists) in `scratch`, a file-scope device array,
+// instead of memory the library allocates, for callers that cannot allocate.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,6 +59,41 @@ __global__ __launch_bounds__(256) void k_bucket(const uint8_t *__restrict__ cid,
     }
 }
 
+// Scratch of ldpc_mixed_decode_noalloc: MIXED_MAX_CODES queue counters, then
+// one frame list of `frames` entries per code.  Each device holds its own copy
+// for the life of the process; host code uses the address that
+// hipGetSymbolAddress returns for the current device, never the name.
+constexpr size_t MIXED_MAX_CODES = 255;
+constexpr size_t MIXED_COUNTS = 256;  // counters, padded to 1 KiB
+static __device__ __align__(256) unsigned scratch[MIXED_COUNTS + LDPC_MIXED_NOALLOC_ENTRIES];
+static_assert(MIXED_MAX_CODES <= MIXED_COUNTS, "one counter per code");
+
+// the device address of `scratch` on the current device, looked up once per device
+constexpr int MAX_DEVICES = 64;
+unsigned *scratch_ptr[MAX_DEVICES] = {};
+std::mutex scratch_mu;
+
+int resolve_scratch(unsigned **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+        ldpc::set_last_error(std::string("mixed scratch: no usable current device: ") + hipGetErrorString(e));
+        return LDPC_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ldpc::set_last_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return LDPC_ERR_HIP;
+        }
+        scratch_ptr[dev] = static_cast<unsigned *>(p);
+    }
+    *out = scratch_ptr[dev];
+    return LDPC_OK;
+}
+
 struct GraphKey {
     const void *in = nullptr, *off = nullptr, *cid = nullptr, *out = nullptr, *iters = nullptr;
     uint64_t frames = 0;
@@ -77,6 +117,8 @@ struct ldpc_mixed {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     GraphKey key;
+    int device = 0;
+    unsigned *scratch = nullptr;  // device address of `scratch` on `device`
 };
 
 #define MHIP(expr)                                                                          \
@@ -123,18 +165,20 @@ int ensure_capacity(ldpc_mixed *m, uint64_t frames) {
     return LDPC_OK;
 }
 
-// the step: clear counters, bucket, fork one frame-queue kernel per code, join
-int enqueue_step(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_off, const uint8_t *d_cid, uint64_t frames,
-                 uint8_t *d_out, int *d_iters, hipStream_t s) {
+// the step: clear counters, bucket, fork one frame-queue kernel per code, join.
+// counts: one queue counter per code; lists: per code `cap` frame ids.
+int enqueue_step(ldpc_mixed *m, unsigned *counts, unsigned *lists, unsigned long long cap, const int8_t *d_in,
+                 const uint64_t *d_off, const uint8_t *d_cid, uint64_t frames, uint8_t *d_out, int *d_iters,
+                 hipStream_t s) {
     const int nc = (int)m->pools.size();
-    MHIP(ldpc::launch_zero_u32(m->d_counts, nc, s));
+    MHIP(ldpc::launch_zero_u32(counts, nc, s));
     // Largest LDS footprint per workgroup first: those workgroups need the
     // most contiguous room on a CU, and a kernel launched first fills the CUs
     // before the next queue starts dispatching (seen on graph replays); the
     // small-footprint kernels then fill the gaps.
     const unsigned grid = (unsigned)std::min<uint64_t>((frames + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_bucket, dim3(grid), dim3(256), 0, s, d_cid, (unsigned long long)frames, nc, m->d_counts,
-                       m->d_lists, m->cap, d_iters);
+    hipLaunchKernelGGL(k_bucket, dim3(grid), dim3(256), 0, s, d_cid, (unsigned long long)frames, nc, counts, lists,
+                       cap, d_iters);
     MHIP(hipGetLastError());
     MHIP(hipEventRecord(m->fork, s));
     for (int c : m->order) {
@@ -143,8 +187,8 @@ int enqueue_step(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_off, const
         io.in = d_in;
         io.out = d_out;
         io.iters = d_iters;
-        io.total_ptr = m->d_counts + c;
-        io.frame_list = m->d_lists + (size_t)c * m->cap;
+        io.total_ptr = counts + c;
+        io.frame_list = lists + (size_t)c * cap;
         io.frame_off = reinterpret_cast<const unsigned long long *>(d_off);
         MHIP(ldpc::pool_enqueue(m->pools[c], io, m->pools[c]->max_grid, m->streams[c]));
         MHIP(hipEventRecord(m->done[c], m->streams[c]));
@@ -199,6 +243,13 @@ ldpc_mixed *ldpc_mixed_create(const ldpc_code *const *codes, int n_codes, int va
         mixed_free(m);
         return nullptr;
     }
+    // set-up of the no-allocation step: this device's copy of `scratch`
+    if (hipGetDevice(&m->device) != hipSuccess || resolve_scratch(&m->scratch) != LDPC_OK) {
+        const std::string why = ldpc_last_error();
+        mixed_free(m);
+        ldpc::set_last_error("ldpc_mixed_create: " + why);
+        return nullptr;
+    }
     return m;
 }
 
@@ -226,7 +277,8 @@ int ldpc_mixed_decode(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_offse
     int rc = ensure_capacity(m, frames);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
-    if (!use_graph) return enqueue_step(m, d_in, d_offsets, d_code_ids, frames, d_out, d_iters, s);
+    if (!use_graph)
+        return enqueue_step(m, m->d_counts, m->d_lists, m->cap, d_in, d_offsets, d_code_ids, frames, d_out, d_iters, s);
 
     GraphKey k;
     k.in = d_in;
@@ -238,7 +290,8 @@ int ldpc_mixed_decode(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_offse
     if (!m->exec || !(k == m->key)) {
         drop_graph(m);
         MHIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_step(m, d_in, d_offsets, d_code_ids, frames, d_out, d_iters, m->stream);
+        rc = enqueue_step(m, m->d_counts, m->d_lists, m->cap, d_in, d_offsets, d_code_ids, frames, d_out, d_iters,
+                          m->stream);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(m->stream, &g);
         if (rc != LDPC_OK) {
@@ -260,6 +313,35 @@ int ldpc_mixed_decode(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_offse
     }
     MHIP(hipGraphLaunch(m->exec, s));
     return LDPC_OK;
+}
+
+int ldpc_mixed_decode_noalloc(ldpc_mixed *m, const int8_t *d_in, const uint64_t *d_offsets,
+                              const uint8_t *d_code_ids, uint64_t frames, uint8_t *d_out, int *d_iters, void *stream) {
+    if (!m || !d_in || !d_offsets || !d_code_ids || !d_out || frames == 0) {
+        ldpc::set_last_error("ldpc_mixed_decode_noalloc: bad arguments");
+        return LDPC_ERR_ARG;
+    }
+    const uint64_t nc = m->pools.size();
+    if (frames > LDPC_MIXED_NOALLOC_ENTRIES / nc) {
+        ldpc::set_last_error("ldpc_mixed_decode_noalloc: frames x codes = " + std::to_string(frames * nc) +
+                             " exceeds the static scratch of " + std::to_string(LDPC_MIXED_NOALLOC_ENTRIES) +
+                             " queue entries (LDPC_MIXED_NOALLOC_ENTRIES)");
+        return LDPC_ERR_ARG;
+    }
+    for (ldpc_pool *p : m->pools)
+        if ((p->variant == LDPC_LOMS || p->variant == LDPC_LNMS) && !p->d_table) {
+            ldpc::set_last_error("learned variant needs a table for every code (ldpc_mixed_set_table)");
+            return LDPC_ERR_STATE;
+        }
+    int dev = -1;
+    MHIP(hipGetDevice(&dev));
+    if (dev != m->device) {
+        ldpc::set_last_error("ldpc_mixed_decode_noalloc: the current device is not the one the object was "
+                             "created on");
+        return LDPC_ERR_STATE;
+    }
+    return enqueue_step(m, m->scratch, m->scratch + MIXED_COUNTS, frames, d_in, d_offsets, d_code_ids, frames, d_out,
+                        d_iters, (hipStream_t)stream);
 }
 
 int ldpc_mixed_config(const ldpc_mixed *m, int code_index, int *words_per_block, int *threads, int *max_workgroups,

@@ -312,6 +312,25 @@ int ldpc_mixed_config(const ldpc_mixed *mixed, int code_index, int *words_per_bl
                       int *max_workgroups, int *uniform_deg, int *batch, int *pairs);
 void ldpc_mixed_destroy(ldpc_mixed *mixed);
 
+/* The mixed-code step without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, semantics and results as ldpc_mixed_decode with
+ * use_graph = 0, but the step's scratch (one queue counter per code and one
+ * frame list of `frames` entries per code) lives in a static device array of
+ * LDPC_MIXED_NOALLOC_ENTRIES + 256 uint32 (8 MiB + 1 KiB) that each GPU holds
+ * until the process ends, instead of memory the object allocates; the call
+ * makes no hipMalloc/hipFree and instantiates no graph.  frames * n_codes
+ * must not exceed LDPC_MIXED_NOALLOC_ENTRIES (else LDPC_ERR_ARG, nothing
+ * enqueued); the current device must be the one the object was created on.
+ * Every such call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (other streams, other objects, other host threads) race and must be
+ * ordered by the caller. */
+#define LDPC_MIXED_NOALLOC_ENTRIES (1u << 21)
+int ldpc_mixed_decode_noalloc(ldpc_mixed *mixed, const int8_t *d_in, const uint64_t *d_offsets,
+                              const uint8_t *d_code_ids, uint64_t frames, uint8_t *d_out, int *d_iters,
+                              void *stream);
+
 /* ---- learned tables: float shadow decoder trained on the GPU ----------------
  * (no counterpart in the reference).  The shadow runs the integer decoder's
  * layered schedule in float LSB units with a per-iteration, per-check-degree
