@@ -62,7 +62,11 @@ def parse():
     ap.add_argument("--table", default=str(TABLE))
     ap.add_argument("--words-per-block", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--cpu-baseline-words", type=int, default=16384)
+    ap.add_argument("--cpu-baseline-words", type=int, default=4096,
+                    help="words (4 frames each) of the all-cores CPU baseline sample")
+    ap.add_argument("--cpu-single-words", type=int, default=128,
+                    help="words of the single-core CPU baseline sample")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU runs after one warm-up (median)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true", help="matched mode: time the learned decoder only")
     ap.add_argument("--traffic-json", default=str(ROOT / "profiles" / "hbm_traffic.json"))
@@ -80,34 +84,83 @@ def traffic_for(path, key):
         return None
 
 
-def cpu_baseline(code, llr, n_words, iters):
-    """The oracle (scalar C restatement of the reference kernel, plain OMS,
-    fixed iterations) on a bounded sample of the same frames, all host cores
-    (capped at 16)."""
+# The GPU host allots this many CPUs to each GPU; a worker pool larger than the
+# share oversubscribes it whatever os.cpu_count() reports.
+CPU_SHARE_PER_GPU = 16
+
+
+def host_cpus():
+    """CPU model and the cores this process may use (affinity, cgroup quota,
+    the per-GPU share), read at run time."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = logical
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return {"model": model, "logical_cpus": logical, "affinity_cpus": affinity, "cgroup_cpu_quota": quota,
+            "share_per_gpu": CPU_SHARE_PER_GPU, "threads": max(1, min(usable, CPU_SHARE_PER_GPU))}
+
+
+def cpu_baseline(code, llr, args, iters):
+    """The oracle (scalar C restatement of the reference kernel: 4 frames per
+    u32, serial layered schedule; plain OMS, fixed iterations) on a bounded
+    sample of the same frames: decode only (the load into the word layout is
+    outside the timing), one warm-up, median of args.cpu_runs runs, on all
+    the cores this process may use, and on one core."""
     from oracle import oracle as O
+    from oracle import build as ob
     deg, pos = code.export()
     N = code.n_vars
-    threads = min(16, os.cpu_count() or 1)
-    sub = llr[: 4 * n_words * N]
-    orc = O.OracleDecoder(deg, pos, N, n_words, O.OMS, q1_split=code.q1_split, early_term=0)
-    orc.load(sub)
-    t0 = time.perf_counter()
-    orc.iterate(iters, threads=threads)
-    dt = time.perf_counter() - t0
-    frames = 4 * n_words
-    # single-core figure on the first 1024 words (4096 frames)
-    w1 = min(1024, n_words)
-    one = O.OracleDecoder(deg, pos, N, w1, O.OMS, q1_split=code.q1_split, early_term=0)
-    one.load(llr[: 4 * w1 * N])
-    t0 = time.perf_counter()
-    one.iterate(iters, threads=1)
-    dt1 = time.perf_counter() - t0
-    return {"value": round(frames * code.n_info / dt / 1e9, 4), "unit": "info Gbit/s", "cores": threads,
+    host = host_cpus()
+    threads = host["threads"]
+
+    def run(words, th):
+        orc = O.OracleDecoder(deg, pos, N, words, O.OMS, q1_split=code.q1_split, early_term=0)
+
+        def once():
+            orc.load(llr[: 4 * words * N])
+            t0 = time.perf_counter()
+            orc.iterate(iters, threads=th)
+            return time.perf_counter() - t0
+        return once
+
+    def timed_decode(words, th):
+        once = run(words, th)
+        once()  # warm-up
+        ts = [once() for _ in range(args.cpu_runs)]
+        return float(np.median(ts)), ts
+
+    w_all, w_one = args.cpu_baseline_words, min(args.cpu_single_words, args.cpu_baseline_words)
+    t_all, ts_all = timed_decode(w_all, threads)
+    t_one, ts_one = timed_decode(w_one, 1)
+    return {"value": round(4 * w_all * code.n_info / t_all / 1e9, 5), "unit": "info Gbit/s", "cores": threads,
             "kind": "port",
-            "sample": f"{frames} frames ({n_words} words) of the same code and input, plain OMS, {iters} fixed "
-                      f"iterations, {dt:.2f} s on {threads} host threads",
-            "single_core": {"value": round(4 * w1 * code.n_info / dt1 / 1e9, 5), "unit": "info Gbit/s",
-                            "sample": f"first {4 * w1} frames, {dt1:.2f} s on 1 thread"}}
+            "sample": f"{4 * w_all} frames ({w_all} words) of the same code and input, plain OMS, {iters} fixed "
+                      f"iterations, decode only; median of {args.cpu_runs} runs after one warm-up, "
+                      f"{t_all:.3f} s on {threads} threads",
+            "runs_s": [round(t, 4) for t in ts_all],
+            "host": host, "compiler": ob.compiler_version(), "flags": " ".join(ob.CFLAGS),
+            "single_core": {"value": round(4 * w_one * code.n_info / t_one / 1e9, 6), "unit": "info Gbit/s",
+                            "sample": f"first {4 * w_one} frames, median of {args.cpu_runs} runs after one "
+                                      f"warm-up, {t_one:.3f} s on 1 thread",
+                            "runs_s": [round(t, 4) for t in ts_one]}}
 
 
 def timed(L, stream, steps, warmup, fn):
@@ -135,9 +188,22 @@ def counts(L, dec, d_hard, frames, n_info):
 
 
 def roofline(alg_bytes, ms, traffic):
+    """HBM roofline of the dominant kernel: algorithmic bytes per launch over
+    its HIP-event time.  With the posterior held in LDS the kernels issue
+    fewer HBM bytes than the algorithmic figure and their limiter is the VALU
+    (rocprofv3 counters in profiles/); frac_measured is the profiled HBM bytes
+    per launch over the same time."""
     achieved = alg_bytes / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": int(alg_bytes),
+           "limiter": "valu"}
+    if traffic:
+        out["frac_measured"] = round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
+
+
+HBM_GBPS_METHOD = ("rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch of this workload (separate counter runs, "
+                   "calibrated on a device copy; profiles/hbm_traffic.json) over this run's kernel time")
 
 
 def run_matched(L, args, t_start):
@@ -153,22 +219,36 @@ def run_matched(L, args, t_start):
     L.channel_generate(d_llr.ptr, N, frames, args.seed, 0, sigma)  # frame-major int8, Philox AWGN
     L.synchronize()
 
-    # learned: frame-queue decoder, per-frame stop, max `iters`
-    pool = L.PoolDecoder(code, variant, args.iters, table=table, words_per_block=args.words_per_block,
-                         threads=args.threads)
+    # learned: frame-queue decoder, per-frame stop, max `iters`; the step is
+    # the captured hipGraph (queue reset + frame-queue kernel), eager beside it
+    probe = L.make_decoder(code, 4, L.OMS)
     d_hard_l, d_it = L.DeviceBuffer(frames * N), L.DeviceBuffer(4 * frames)
-    ms_l = timed(L, pool.stream, args.steps, args.warmup,
-                 lambda: pool.decode_device(d_llr.ptr, frames, d_hard_l.ptr, d_it.ptr))
-    its = d_it.download(np.empty(frames, np.int32))
-    learned = counts(L, L.make_decoder(code, 4, L.OMS), d_hard_l.ptr, frames, K)
-    iters_sum = int(its.sum(dtype=np.int64))
+
+    def queue_leg(v, tab):
+        pool = L.PoolDecoder(code, v, args.iters, table=tab, words_per_block=args.words_per_block,
+                             threads=args.threads)
+        ms_eager = timed(L, pool.stream, args.steps, args.warmup,
+                         lambda: pool.decode_device(d_llr.ptr, frames, d_hard_l.ptr, d_it.ptr))
+        ms = timed(L, pool.stream, args.steps, args.warmup,
+                   lambda: pool.decode_device(d_llr.ptr, frames, d_hard_l.ptr, d_it.ptr, graph=True))
+        its = d_it.download(np.empty(frames, np.int32))
+        r = counts(L, probe, d_hard_l.ptr, frames, K)
+        it_sum = int(its.sum(dtype=np.int64))
+        r.update({"info_gbps": round(frames * K / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 4),
+                  "graph_ms_per_step": round(ms, 4), "eager_ms_per_step": round(ms_eager, 4),
+                  "avg_iterations": round(it_sum / frames, 4), "max_iterations": args.iters,
+                  "decoder": "frame-queue (persistent, per-frame stop, refill), step replayed from a hipGraph"})
+        r.update(pool.config())
+        return r, ms, it_sum
+
+    learned, ms_l, iters_sum = queue_leg(variant, table)
+    learned["table_deg6"] = [int(x) for x in table[:, min(6, table.shape[1] - 1)]]
     alg_l = iters_sum * (2 * E + 2 * N) + frames * 2 * N
     gbps_l = frames * K / (ms_l * 1e-3) / 1e9
-    learned.update({"info_gbps": round(gbps_l, 3), "ms_per_step": round(ms_l, 4),
-                    "avg_iterations": round(iters_sum / frames, 4), "max_iterations": args.iters,
-                    "decoder": "frame-queue (persistent, per-frame stop, refill)",
-                    "table_deg6": [int(x) for x in table[:, min(6, table.shape[1] - 1)]]})
-    learned.update(pool.config())
+    # plain OMS in the same frame queue on the same frames: what the table alone gains
+    plain_queue = None
+    if not args.skip_plain:
+        plain_queue, _, _ = queue_leg(L.OMS, None)
     del d_hard_l
 
     plain = None
@@ -206,19 +286,29 @@ def run_matched(L, args, t_start):
         "config": {"workload": "learned-offset OMS vs plain 20-iteration OMS, (3,6) N=4000 K=2000, "
                                "%d frames resident in HBM, Eb/N0 %.2f dB" % (frames, args.ebn0),
                    "frames": frames, "ebn0_db": args.ebn0, "parallelism": "single GPU",
-                   "learned": learned, "plain": plain},
+                   "graph_ms_per_step": learned["graph_ms_per_step"],
+                   "eager_ms_per_step": learned["eager_ms_per_step"],
+                   "learned": learned, "plain": plain, "plain_frame_queue": plain_queue},
         "roofline": roofline(alg_l, ms_l, traffic_for(args.traffic_json, key)),
     }
     tr = out["roofline"]["traffic"]
-    if tr:  # the metric's "HBM GB/s": profiled bytes per launch over this run's kernel time
+    if tr:  # the metric's "HBM GB/s"
         out["config"]["hbm_gbps"] = round(tr / (ms_l * 1e-3) / 1e9, 1)
+        out["config"]["hbm_gbps_method"] = HBM_GBPS_METHOD
     if plain:
+        # learned frame queue vs plain drop-in at 20 fixed iterations: the
+        # per-frame stop and the table together
         out["config"]["speedup_vs_plain"] = round(gbps_l / plain["info_gbps"], 3)
         out["config"]["fer_matched"] = learned["fer"] <= plain["fer"]
+    if plain_queue:
+        # the table alone: learned vs plain OMS in the same frame queue
+        out["config"]["speedup_vs_plain_frame_queue"] = round(gbps_l / plain_queue["info_gbps"], 3)
+        out["config"]["speedup_parts"] = ("speedup_vs_plain = per-frame stop x table; "
+                                          "speedup_vs_plain_frame_queue = the table alone")
     if not args.no_cpu_baseline:
         n = 4 * args.cpu_baseline_words * N
         llr = d_llr.download(np.empty(n, np.int8))
-        out["cpu_baseline"] = cpu_baseline(code, llr, args.cpu_baseline_words, args.iters)
+        out["cpu_baseline"] = cpu_baseline(code, llr, args, args.iters)
     out["wall_s"] = round(time.perf_counter() - t_start, 3)
     return out
 
@@ -279,12 +369,13 @@ def run_plain(L, args, t_start):
                              if variant == L.OMS else None),
     }
     tr = out["roofline"]["traffic"]
-    if tr:  # profiled HBM bytes per launch over this run's kernel time
+    if tr:
         out["config"]["hbm_gbps"] = round(tr / (ms * 1e-3) / 1e9, 1)
+        out["config"]["hbm_gbps_method"] = HBM_GBPS_METHOD
     if not args.no_cpu_baseline and variant == L.OMS:
         n = 4 * args.cpu_baseline_words * N
         llr = d_in.download(np.empty(n, np.int8))
-        out["cpu_baseline"] = cpu_baseline(code, llr, args.cpu_baseline_words, args.iters)
+        out["cpu_baseline"] = cpu_baseline(code, llr, args, args.iters)
     out["wall_s"] = round(time.perf_counter() - t_start, 3)
     return out
 
@@ -405,23 +496,33 @@ def run_mixed(L, args, t_start):
     }
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        threads = min(16, os.cpu_count() or 1)
-        t_cpu, bits = 0.0, 0
+        from oracle import build as ob
+        host = host_cpus()
+        threads = host["threads"]
+        nf = 1024
+        samples, bits = [], 0
+        raw = d_in.download(np.empty(base[1] + nf * codes[1].n_vars, np.int8))
         for k, c in enumerate(codes):
-            nf = 2048
-            llr = d_in.download(np.empty(nf * c.n_vars, np.int8)) if k == 0 else None
-            if k == 1:
-                raw = np.empty(base[1] + nf * c.n_vars, np.int8)
-                llr = d_in.download(raw)[base[1]:]
             deg, pos = c.export()
-            t0 = time.perf_counter()
-            O.decode_frames(deg, pos, c.n_vars, O.OMS_V2, llr.reshape(nf, c.n_vars), max_it, threads=threads)
-            t_cpu += time.perf_counter() - t0
+            llr = raw[base[k]:base[k] + nf * c.n_vars].reshape(nf, c.n_vars)
+            samples.append((deg, pos, c.n_vars, llr))
             bits += nf * c.n_info
-        out["cpu_baseline"] = {"value": round(bits / t_cpu / 1e9, 4), "unit": "info Gbit/s", "cores": threads,
+
+        def once():
+            t0 = time.perf_counter()
+            for deg, pos, n, llr in samples:
+                O.decode_frames(deg, pos, n, O.OMS_V2, llr, max_it, threads=threads)
+            return time.perf_counter() - t0
+        once()  # warm-up
+        ts = [once() for _ in range(args.cpu_runs)]
+        t_cpu = float(np.median(ts))
+        out["cpu_baseline"] = {"value": round(bits / t_cpu / 1e9, 5), "unit": "info Gbit/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"2048 frames of each code from the same input, per-frame stop, max {max_it}, "
-                                         f"{t_cpu:.2f} s on {threads} host threads"}
+                               "sample": f"{nf} frames of each code from the same input, per-frame stop, max {max_it}; "
+                                         f"median of {args.cpu_runs} runs after one warm-up, {t_cpu:.3f} s on "
+                                         f"{threads} threads",
+                               "runs_s": [round(t, 4) for t in ts], "host": host,
+                               "compiler": ob.compiler_version(), "flags": " ".join(ob.CFLAGS)}
     out["wall_s"] = round(time.perf_counter() - t_start, 3)
     return out
 

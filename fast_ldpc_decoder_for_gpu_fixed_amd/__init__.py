@@ -392,10 +392,11 @@ class PoolDecoder:
         check(load().ldpc_pool_set_table(self._h, ptr(t, C.c_int8), t.shape[0], t.shape[1]), "pool_set_table")
 
     def decode_device(self, d_in: int, frames: int, d_out: int, d_iters: int = 0, workgroups: int = 0,
-                      stream: int = 0) -> None:
-        check(load().ldpc_pool_decode(self._h, C.c_void_p(d_in), frames, C.c_void_p(d_out),
-                                      C.c_void_p(d_iters) if d_iters else None, workgroups,
-                                      C.c_void_p(stream) if stream else None), "pool_decode")
+                      stream: int = 0, graph: bool = False) -> None:
+        """graph=True: the step captured in a hipGraph once per buffer set, then replayed."""
+        fn = load().ldpc_pool_decode_graph if graph else load().ldpc_pool_decode
+        check(fn(self._h, C.c_void_p(d_in), frames, C.c_void_p(d_out), C.c_void_p(d_iters) if d_iters else None,
+                 workgroups, C.c_void_p(stream) if stream else None), "pool_decode_graph" if graph else "pool_decode")
 
     def decode(self, llr: np.ndarray, workgroups: int = 0):
         """Host convenience: frame-major int8 LLRs -> (decisions [F][N], iterations [F])."""

@@ -1,7 +1,6 @@
-"""Build the gfx950 shared library (and, for the tests, the CPU oracle).
+"""Build the gfx950 shared library and the sweep driver.
 
-    python -m fast_ldpc_decoder_for_gpu_fixed_amd.build      # both
-    python -m fast_ldpc_decoder_for_gpu_fixed_amd.build lib  # product only
+    python -m fast_ldpc_decoder_for_gpu_fixed_amd.build
 
 The product is ``libldpc_mi355x.so`` next to this file: HIP kernels for gfx950
 plus the C-ABI of ``include/ldpc_mi355x.h``.  Objects are compiled in
@@ -72,19 +71,8 @@ def build_sim() -> Path:
     return SIM
 
 
-def build_oracle() -> Path:
-    """The CPU oracle is test infrastructure (oracle/); built here for convenience."""
-    sys.path.insert(0, str(ROOT))
-    from oracle import build as ob  # noqa: E402
-    return ob.build()
-
-
 def main(argv: list[str]) -> None:
-    what = argv[1] if len(argv) > 1 else "all"
-    if what in ("all", "lib"):
-        print("built", build_lib())
-    if what in ("all", "oracle"):
-        print("built", build_oracle())
+    print("built", build_lib())
 
 
 if __name__ == "__main__":

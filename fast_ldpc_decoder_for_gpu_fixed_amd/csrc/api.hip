@@ -439,7 +439,10 @@ int ldpc_decoder_config(const ldpc_decoder *dec, int *words_per_block, int *thre
 }
 
 int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, int n_degs) {
-    if (!dec || !table || n_iters < 1 || n_degs < 1) return fail(LDPC_ERR_ARG, "bad table");
+    // the frame-queue decoder's limits (long frames run on one), checked
+    // before any state changes
+    if (!dec || !table || n_iters < 1 || n_degs < 1 || n_iters > 64 || n_degs > ldpc::MAX_DEG + 1)
+        return fail(LDPC_ERR_ARG, "ldpc_decoder_set_table: bad table (at most 64 rows, 33 columns)");
     const int lo = 0, hi = dec->variant == LDPC_LNMS ? 8 : 127;
     for (int i = 0; i < n_iters * n_degs; i++)
         if (table[i] < lo || table[i] > hi) return fail(LDPC_ERR_ARG, "table value out of range");
@@ -451,7 +454,13 @@ int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, 
     dec->tits = n_iters;
     dec->tdeg = n_degs;
     dec->h_table.assign(table, table + (size_t)n_iters * n_degs);
-    if (dec->long_pool) return ldpc_pool_set_table(dec->long_pool, table, n_iters, n_degs);
+    if (dec->long_pool && ldpc_pool_set_table(dec->long_pool, table, n_iters, n_degs) != LDPC_OK) {
+        // the pair block no longer fits beside this table: drop the frame
+        // queue, long frames then use the global-memory-posterior kernel
+        ldpc_pool_destroy(dec->long_pool);
+        dec->long_pool = nullptr;
+        dec->long_frames = false;
+    }
     return LDPC_OK;
 }
 
@@ -473,9 +482,11 @@ static int decode_long(ldpc_decoder *dec, const int8_t *d_in, uint8_t *d_out, in
         const ldpc_tuning tu = {0, 0, -1, 2, 0, 0};  // pairs, automatic block and workgroup size
         dec->long_pool = ldpc_pool_create(&dec->code, dec->variant, iterations, LDPC_STOP_NONE, &tu);
         if (!dec->long_pool) return LDPC_ERR_ARG;  // (the caller falls back to the global-memory kernel)
-        if (!dec->h_table.empty()) {
-            const int rc = ldpc_pool_set_table(dec->long_pool, dec->h_table.data(), dec->tits, dec->tdeg);
-            if (rc) return rc;
+        if (!dec->h_table.empty() &&
+            ldpc_pool_set_table(dec->long_pool, dec->h_table.data(), dec->tits, dec->tdeg) != LDPC_OK) {
+            ldpc_pool_destroy(dec->long_pool);  // no room beside the table: the global-memory kernel
+            dec->long_pool = nullptr;
+            return LDPC_ERR_ARG;
         }
     }
     dec->long_pool->max_iters = iterations;

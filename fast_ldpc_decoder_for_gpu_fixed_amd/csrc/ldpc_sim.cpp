@@ -22,9 +22,7 @@
 // the frame errors reach -fer (CErrorAnalyzer::fe_limit_achieved,
 // src/ber_analyzer/CErrorAnalyzer.cpp:113-116), the frame cap or the timer.
 // Printed per point: the reference's final-report fields plus information
-// throughput from the decode time alone and the algorithmic HBM rate; while
-// a point runs, a progress line once a second (CTerminal::temp_report,
-// src/terminal/CTerminal.cpp).
+// throughput from the decode time alone and the algorithmic HBM rate.
 #include "ldpc_mi355x.h"
 
 #include <chrono>
@@ -163,7 +161,6 @@ int main(int argc, char **argv) {
         unsigned long long frames = 0, fe = 0, be = 0, k = 0;
         double decode_ms = 0.0, it_sum = 0.0;
         bool timed_out = false;
-        auto last_report = t_point;
         while (fe < (unsigned long long)o.fe_limit && (!o.max_frames || frames < o.max_frames)) {
             ldpc_sim_stats st;
             const unsigned long long seed = o.seed + 0x9E3779B97F4A7C15ull * (k + 1);
@@ -179,21 +176,6 @@ int main(int argc, char **argv) {
             decode_ms += st.decode_ms;
             it_sum += st.avg_iterations * (double)st.frames;
             k++;
-            const auto now = std::chrono::steady_clock::now();
-            if (std::chrono::duration<double>(now - last_report).count() >= 1.0) {
-                last_report = now;
-                const double t = std::chrono::duration<double>(now - t_point).count();
-                std::printf("(RT) FRA: %10llu | FE: %6llu | FER: %2.2e | BE: %8llu | BER: %2.2e | ", frames, fe,
-                            (double)fe / (double)frames, be, (double)be / (double)frames / (double)K);
-                show_time((unsigned long)t);
-                std::printf(" | ETR: ");
-                if (fe)
-                    show_time((unsigned long)(t / (double)fe * (double)(o.fe_limit > (int)fe ? o.fe_limit - fe : 0)));
-                else
-                    std::printf("INF.");
-                std::printf("\r");
-                std::fflush(stdout);
-            }
             if (o.timer_s != -1 && elapsed_s() >= o.timer_s) {
                 timed_out = true;
                 break;

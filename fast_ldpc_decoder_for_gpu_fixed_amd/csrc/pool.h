@@ -24,6 +24,15 @@ struct ldpc_pool {
     int pairs = 0;  // posterior and messages as 2-frame pairs (P counts pairs)
     uint32_t *d_rows = nullptr;  // pair mode with degrees <= 8: ids as rows of 8 per check
     int n_cu = 1;   // compute units of the device
+    // captured decode step (ldpc_pool_decode_graph) and what it was captured with
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    struct {
+        const void *in, *out, *iters, *table;
+        unsigned long long frames;
+        int grid, max_iters;
+    } key = {};
 };
 
 

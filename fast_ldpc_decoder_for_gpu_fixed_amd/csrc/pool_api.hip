@@ -25,8 +25,18 @@ void set_last_error(const std::string &m);
     } while (0)
 
 namespace {
+void drop_graph(ldpc_pool *p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+    p->key = {};
+}
+
 void pool_free(ldpc_pool *p) {
     if (!p) return;
+    drop_graph(p);
+    if (p->capture) (void)hipStreamDestroy(p->capture);
     (void)hipFree(p->d_checks);
     (void)hipFree(p->d_edge_var);
     (void)hipFree(p->d_level_beg);
@@ -246,6 +256,7 @@ int ldpc_pool_set_table(ldpc_pool *p, const int8_t *table, int n_iters, int n_de
             return LDPC_ERR_ARG;
         }
     PHIP(hipStreamSynchronize(p->stream));
+    drop_graph(p);  // a captured step holds the old table's address
     (void)hipFree(p->d_table);
     p->d_table = nullptr;
     PHIP(hipMalloc((void **)&p->d_table, (size_t)n_iters * n_degs));
@@ -263,26 +274,82 @@ int ldpc_pool_set_table(ldpc_pool *p, const int8_t *table, int n_iters, int n_de
     return LDPC_OK;
 }
 
-int ldpc_pool_decode(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters, int workgroups,
-                     void *stream) {
+static int pool_decode_args(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int workgroups,
+                            const char *who, int *grid) {
     if (!p || !d_in || !d_out || frames == 0) {
-        ldpc::set_last_error("ldpc_pool_decode: bad arguments");
+        ldpc::set_last_error(std::string(who) + ": bad arguments");
         return LDPC_ERR_ARG;
     }
     if ((p->variant == LDPC_LOMS || p->variant == LDPC_LNMS) && !p->d_table) {
         ldpc::set_last_error("learned variant needs ldpc_pool_set_table first");
         return LDPC_ERR_STATE;
     }
-    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
     const uint64_t slots = (p->pairs ? 2ull : 4ull) * (uint64_t)p->P;
-    int grid = workgroups > 0 ? std::min(workgroups, p->max_grid) : p->max_grid;
-    grid = (int)std::min<uint64_t>((uint64_t)grid, (frames + slots - 1) / slots);
+    const int g = workgroups > 0 ? std::min(workgroups, p->max_grid) : p->max_grid;
+    *grid = (int)std::min<uint64_t>((uint64_t)g, (frames + slots - 1) / slots);
+    return LDPC_OK;
+}
+
+int ldpc_pool_decode(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters, int workgroups,
+                     void *stream) {
+    int grid = 0;
+    const int rc = pool_decode_args(p, d_in, frames, d_out, workgroups, "ldpc_pool_decode", &grid);
+    if (rc) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
     ldpc::PoolIO io;
     io.in = d_in;
     io.out = d_out;
     io.iters = d_iters;
     io.total = frames;
     PHIP(ldpc::pool_enqueue(p, io, grid, s));
+    return LDPC_OK;
+}
+
+int ldpc_pool_decode_graph(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters,
+                           int workgroups, void *stream) {
+    int grid = 0;
+    int rc = pool_decode_args(p, d_in, frames, d_out, workgroups, "ldpc_pool_decode_graph", &grid);
+    if (rc) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    const bool same = p->exec && p->key.in == d_in && p->key.out == d_out && p->key.iters == d_iters &&
+                      p->key.table == p->d_table && p->key.frames == frames && p->key.grid == grid &&
+                      p->key.max_iters == p->max_iters;
+    if (!same) {
+        // capture once: the queue-head reset kernel and the decode kernel
+        // (kernels only, no memset node), on the pool's capture stream
+        drop_graph(p);
+        if (!p->capture) PHIP(hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking));
+        PHIP(hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal));
+        ldpc::PoolIO io;
+        io.in = d_in;
+        io.out = d_out;
+        io.iters = d_iters;
+        io.total = frames;
+        const hipError_t el = ldpc::pool_enqueue(p, io, grid, p->capture);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(p->capture, &g);
+        if (el != hipSuccess || ec != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            ldpc::set_last_error(std::string("ldpc_pool_decode_graph: capture: ") +
+                                 hipGetErrorString(el != hipSuccess ? el : ec));
+            return LDPC_ERR_HIP;
+        }
+        p->graph = g;
+        const hipError_t ei = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            drop_graph(p);
+            ldpc::set_last_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            return LDPC_ERR_HIP;
+        }
+        p->key.in = d_in;
+        p->key.out = d_out;
+        p->key.iters = d_iters;
+        p->key.table = p->d_table;
+        p->key.frames = frames;
+        p->key.grid = grid;
+        p->key.max_iters = p->max_iters;
+    }
+    PHIP(hipGraphLaunch(p->exec, s));
     return LDPC_OK;
 }
 

@@ -181,6 +181,12 @@ public:
                        void *stream = nullptr, int workgroups = 0) {
         ldpc::check(ldpc_pool_decode(p_, d_in, frames, d_out, d_iters, workgroups, stream), "ldpc_pool_decode");
     }
+    // the same step captured in a hipGraph once per buffer set and replayed
+    void decode_graph(const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters = nullptr,
+                      void *stream = nullptr, int workgroups = 0) {
+        ldpc::check(ldpc_pool_decode_graph(p_, d_in, frames, d_out, d_iters, workgroups, stream),
+                    "ldpc_pool_decode_graph");
+    }
     ldpc_sim_stats simulate(double ebn0_db, uint64_t frames, uint64_t seed = 1, uint32_t snr_idx = 0, int quant = 0,
                             uint64_t batch_frames = 0) {
         ldpc_sim_stats s;

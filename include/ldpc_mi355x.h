@@ -158,7 +158,8 @@ int ldpc_decoder_config(const ldpc_decoder *dec, int *words_per_block, int *thre
 /* Learned table for LDPC_LOMS / LDPC_LNMS: int8 [n_iters][n_degs], row =
  * absolute iteration (last row repeats), column = check degree (last column
  * repeats).  LOMS: offset in LSB, 0..127. LNMS: numerator over 8, 0..8,
- * magnitude capped at 31. */
+ * magnitude capped at 31.  At most 64 rows and 33 columns (the frame-queue
+ * decoder's limits; a rejected table leaves the decoder unchanged). */
 int ldpc_decoder_set_table(ldpc_decoder *dec, const int8_t *table, int n_iters, int n_degs);
 
 /* ---- state API (per-iteration parity; the reference keeps this state in
@@ -266,6 +267,14 @@ int ldpc_pool_set_table(ldpc_pool *pool, const int8_t *table, int n_iters, int n
  * frames = 0 is LDPC_ERR_ARG (as is an empty step of ldpc_mixed_decode). */
 int ldpc_pool_decode(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters,
                      int workgroups, void *stream);
+/* The same decode step captured in a hipGraph (the reset kernel and the
+ * frame-queue kernel) on first use and replayed on `stream` while the buffer
+ * arguments, the frame count, the grid, the table and max_iterations stay the
+ * same; a change recaptures (the per-batch decode() call of the reference's
+ * simulation loop, src/main.cpp:266-268, as a replayed graph).  Results equal
+ * ldpc_pool_decode's; the same no-overlap rule holds. */
+int ldpc_pool_decode_graph(ldpc_pool *pool, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters,
+                           int workgroups, void *stream);
 /* One Eb/N0 point: device channel (frames 0 .. n_frames-1 of the stream
  * (seed, snr_idx), the same frames ldpc_simulate draws) -> pool decode ->
  * device error counting, batch_frames frames per launch (0 = 2^18).

@@ -14,10 +14,22 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "libldpc_oracle.so"
 
 
+CC = os.environ.get("CC", "gcc")
+CFLAGS = ["-O3", "-std=c11", "-fPIC", "-shared", "-pthread", "-Wall", "-Wextra"]
+
+
+def compiler_version() -> str:
+    """First line of the C compiler's --version (recorded beside CPU timings)."""
+    try:
+        r = subprocess.run([CC, "--version"], capture_output=True, text=True)
+        return r.stdout.splitlines()[0].strip() if r.returncode == 0 and r.stdout else CC
+    except OSError:
+        return CC
+
+
 def build() -> Path:
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [os.environ.get("CC", "gcc"), "-O2", "-std=c11", "-fPIC", "-shared", "-pthread",
-           "-Wall", "-Wextra", str(HERE / "ldpc_oracle.c"), "-o", str(tmp), "-lm"]
+    cmd = [CC, *CFLAGS, str(HERE / "ldpc_oracle.c"), "-o", str(tmp), "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}{r.stderr}")

@@ -333,3 +333,11 @@ def test_error_paths(gpu_available):
         learned.decode_bytes(llr, 5)  # no table yet
     with pytest.raises(L.LdpcError):
         learned.set_table(np.full((2, 7), -1, np.int8))
+    # a table beyond the frame queue's limits is rejected and the decoder keeps its table
+    learned.set_table(learned_table(L.LOMS))
+    before = learned.decode_bytes(llr, 5)
+    with pytest.raises(L.LdpcError):
+        learned.set_table(np.zeros((65, 33), np.int8))
+    with pytest.raises(L.LdpcError):
+        learned.set_table(np.zeros((4, 34), np.int8))
+    np.testing.assert_array_equal(learned.decode_bytes(llr, 5), before)

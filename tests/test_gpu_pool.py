@@ -241,3 +241,51 @@ def test_pool_rejects_blocks_beyond_lds(gpu_available):
     with pytest.raises(L.LdpcError):
         L.PoolDecoder(c, L.OMS_V2, 10, pairs=0)
     assert L.PoolDecoder(c, L.OMS_V2, 10).config()["pairs"] == 1  # auto: pair mode
+
+
+def test_pool_graph_replays_match_eager_and_oracle(gpu_available):
+    """ldpc_pool_decode_graph: the captured step (queue-head reset kernel +
+    frame-queue kernel) replayed several times, with eager work and
+    synchronisations between replays, equals the eager decode and the oracle;
+    new buffers, a new table or another frame count recapture."""
+    c = L.code_3_6(4000, seed=1)
+    deg, pos = c.export()
+    F = 4096
+    llr = mixed_llr(c.n_vars, F, seed=31, lo=1.5, hi=2.5)
+    table = table_for(L.LOMS)
+    pool = L.PoolDecoder(c, L.LOMS, 20, table=table)
+    ref_h, ref_i = O.decode_frames(deg, pos, c.n_vars, L.LOMS, llr, 20, table=table, threads=THREADS)
+    d_in, d_out, d_it = L.DeviceBuffer(llr.nbytes), L.DeviceBuffer(llr.nbytes), L.DeviceBuffer(4 * F)
+    d_in.upload(llr)
+
+    def run(graph, out=d_out, it=d_it, frames=F):
+        out.upload(np.full(llr.nbytes, 0xAA, np.uint8))
+        it.upload(np.full(F, -1, np.int32))
+        pool.decode_device(d_in.ptr, frames, out.ptr, it.ptr, graph=graph)
+        L.synchronize()
+        return out.download(np.empty((F, c.n_vars), np.uint8)), it.download(np.empty(F, np.int32))
+
+    for rep in range(3):
+        h, i = run(graph=True)
+        np.testing.assert_array_equal(i, ref_i, err_msg=f"graph replay {rep}")
+        np.testing.assert_array_equal(h, ref_h, err_msg=f"graph replay {rep}")
+        h, i = run(graph=False)
+        np.testing.assert_array_equal(i, ref_i, err_msg=f"eager after replay {rep}")
+        np.testing.assert_array_equal(h, ref_h)
+    # other output buffers: recaptured
+    o2, i2 = L.DeviceBuffer(llr.nbytes), L.DeviceBuffer(4 * F)
+    h, i = run(graph=True, out=o2, it=i2)
+    np.testing.assert_array_equal(i, ref_i)
+    np.testing.assert_array_equal(h, ref_h)
+    # fewer frames: recaptured, the rest of the buffer untouched
+    h, i = run(graph=True, frames=F // 2)
+    np.testing.assert_array_equal(i[:F // 2], ref_i[:F // 2])
+    np.testing.assert_array_equal(h[:F // 2], ref_h[:F // 2])
+    assert (i[F // 2:] == -1).all() and (h[F // 2:] == 0xAA).all()
+    # a new table: recaptured with it
+    t2 = table_for(L.LOMS, seed=6)
+    pool.set_table(t2)
+    r2_h, r2_i = O.decode_frames(deg, pos, c.n_vars, L.LOMS, llr, 20, table=t2, threads=THREADS)
+    h, i = run(graph=True)
+    np.testing.assert_array_equal(i, r2_i)
+    np.testing.assert_array_equal(h, r2_h)
