@@ -97,8 +97,13 @@ __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 
         const u16x2 y = sub_sat_s(zero, v);  // -t, top bit = (t > 0)
         const u16x2 a = umin(v, y);          // |t| as uint
         sg ^= y;
-        m2 = umin(m2, umax(a, m1));
-        m1 = umin(m1, a);
+        if (j == 0) {
+            // min2 = min(127, max(a, 127)) = 127 after the first edge
+            m1 = umin(m1, a);
+        } else {
+            m2 = umin(m2, umax(a, m1));
+            m1 = umin(m1, a);
+        }
         if constexpr (KEEP) {
             ng[j] = y;
             ab[j] = a;
@@ -145,21 +150,39 @@ __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 
     return parity;
 }
 
+// How a word's message bytes are unpacked: the even half by v_perm with
+// `sel_even`, the odd half by AND with `mask_odd`.  A byte lane whose frame
+// reads zero messages (a frame that starts now in the frame-queue decoder)
+// gets the zero selector 0x0C in the even half and a zero mask byte in the
+// odd half, so clearing it costs nothing beyond the unpack itself.
+struct MsgUnpack {
+    uint32_t sel_even = 0x020C000Cu, mask_odd = 0xFF00FF00u;
+};
+// keep: 0xFF for each byte lane whose messages are read, 0x00 for a lane
+// whose messages count as zero
+__device__ __forceinline__ MsgUnpack msg_unpack(uint32_t keep) {
+    MsgUnpack u;
+    if (!(keep & 0x000000FFu)) u.sel_even = (u.sel_even & 0xFFFF00FFu) | 0x00000C00u;
+    if (!(keep & 0x00FF0000u)) u.sel_even = (u.sel_even & 0x00FFFFFFu) | 0x0C000000u;
+    u.mask_odd = keep & 0xFF00FF00u;
+    return u;
+}
+
 // one check of degree D on one word (4 frames = even + odd half).
 // Returns the parity bits of the even (x) and odd (y) halves at 0x80008000:
 // frame 0 = x bit 15, 1 = y bit 15, 2 = x bit 31, 3 = y bit 31.
 template <int D, bool EQ>
 __device__ __forceinline__ uint2 check_core(const uint32_t (&pw)[D], const uint32_t (&mw)[D], bool first,
                                             int variant, bool nocap, u16x2 lpe, u16x2 lpo, uint32_t (&pn)[D],
-                                            uint32_t (&mn)[D]) {
+                                            uint32_t (&mn)[D], MsgUnpack mu = MsgUnpack()) {
     u16x2 xe[D], xo[D], me[D], mo[D], pe[D], po[D], ne[D], no[D];
 #pragma unroll
     for (int j = 0; j < D; j++) {
         xe[j] = unpack_even(pw[j]);
         xo[j] = unpack_odd(pw[j]);
         if (!first) {
-            me[j] = unpack_even(mw[j]);
-            mo[j] = unpack_odd(mw[j]);
+            me[j] = u2(__builtin_amdgcn_perm(mw[j], mw[j], mu.sel_even));
+            mo[j] = u2(mw[j] & mu.mask_odd);
         }
     }
     const uint32_t pe_bits = check_half<D, EQ>(xe, me, first, variant, nocap, lpe, pe, ne);
@@ -206,8 +229,8 @@ __device__ __forceinline__ void store_msgs(uint32_t *__restrict__ mp, const uint
 // start at c*D, its LDS byte offsets (v * P * 4) come packed two per dword
 // (edge_off16), and the index and message loads of G items are issued
 // together before any of them is computed.  FIRST: no message read (the
-// decode's first iteration).  keep: byte mask applied to every message word
-// read (a lane whose frame starts now reads zero messages).  VK >= 0: the
+// decode's first iteration).  keep: byte mask of the message lanes read (a
+// lane whose frame starts now reads zero messages; folded into the unpack).  VK >= 0: the
 // variant known at compile time (its magnitude map is then straight-line code
 // with no per-item branch), -1: read from `variant`.  Returns the OR of the
 // items' parity bits (check_core's layout).
@@ -219,6 +242,7 @@ __device__ __forceinline__ uint2 uniform_items(const uint32_t *__restrict__ edge
     constexpr int H = (D + 1) / 2;
     if constexpr (VK >= 0) variant = VK;
     char *const pbase = reinterpret_cast<char *>(post) + 4 * w;
+    const MsgUnpack mu = msg_unpack(keep);
     uint2 ov = make_uint2(0u, 0u);
     for (int c0 = c_first; c0 < le; c0 += G * cstride) {
         uint32_t off[G][H];
@@ -229,11 +253,7 @@ __device__ __forceinline__ uint2 uniform_items(const uint32_t *__restrict__ edge
             if (c < le) {
 #pragma unroll
                 for (int h = 0; h < H; h++) off[g][h] = edge_off16[(size_t)c * H + h];
-                if (!FIRST) {
-                    load_msgs<D>(mbase + ((size_t)c * P + w) * D, mw[g]);
-#pragma unroll
-                    for (int j = 0; j < D; j++) mw[g][j] &= keep;
-                }
+                if (!FIRST) load_msgs<D>(mbase + ((size_t)c * P + w) * D, mw[g]);
             }
         }
 #pragma unroll
@@ -248,7 +268,7 @@ __device__ __forceinline__ uint2 uniform_items(const uint32_t *__restrict__ edge
                         pbase + ((j & 1) ? (off[g][j >> 1] >> 16) : (off[g][j >> 1] & 0xFFFFu)));
 #pragma unroll
                 for (int j = 0; j < D; j++) pw[j] = *vp[j];
-                const uint2 pr = check_core<D, EQ>(pw, mw[g], FIRST, variant, false, lpe, lpo, pn, mn);
+                const uint2 pr = check_core<D, EQ>(pw, mw[g], FIRST, variant, false, lpe, lpo, pn, mn, mu);
                 ov.x |= pr.x;
                 ov.y |= pr.y;
                 store_msgs<D>(mbase + ((size_t)c * P + w) * D, mn);

@@ -29,8 +29,8 @@ __device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__re
 #pragma unroll
     for (int j = 0; j < D; j++) pw[j] = post[vi[j]];
 #pragma unroll
-    for (int j = 0; j < D; j++) mw[j] = mp[j] & keep;
-    const uint2 pr = check_core<D, EQ>(pw, mw, false, variant, false, lpe, lpo, pn, mn);
+    for (int j = 0; j < D; j++) mw[j] = mp[j];
+    const uint2 pr = check_core<D, EQ>(pw, mw, false, variant, false, lpe, lpo, pn, mn, msg_unpack(keep));
 #pragma unroll
     for (int j = 0; j < D; j++) {
         mp[j] = mn[j];
@@ -45,6 +45,11 @@ __device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__re
 // result are junk: XOR-accumulate, then keep 0x80808080 once per check.
 __device__ __forceinline__ uint32_t positive_msb(uint32_t x) {
     return ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) & ~x;
+}
+// the same on two words at once: no carry crosses a byte, so one 64-bit add
+// (v_lshl_add_u64) does the work of two 32-bit adds
+__device__ __forceinline__ uint64_t positive_msb64(uint64_t x) {
+    return ((x & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) & ~x;
 }
 
 // syndrome of the decisions over every check, per byte (bit 7 of each byte
@@ -72,17 +77,15 @@ __device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_
                 }
 #pragma unroll
                 for (int b = 0; b < SB; b++) {
-                    uint32_t p0 = 0, p1 = 0;
+                    uint64_t pp = 0;
 #pragma unroll
                     for (int j = 0; j < UD; j++) {
                         const uint32_t o = (j & 1) ? (off[b][j >> 1] >> 16) : (off[b][j >> 1] & 0xFFFFu);
-                        const uint2 x = *reinterpret_cast<const uint2 *>(pbase + o);
-                        p0 ^= positive_msb(x.x);
-                        p1 ^= positive_msb(x.y);
+                        pp ^= positive_msb64(*reinterpret_cast<const uint64_t *>(pbase + o));
                     }
                     // a clamped index repeats check C-1: OR-ing its parity twice is harmless
-                    synd.x |= p0;
-                    synd.y |= p1;
+                    synd.x |= (uint32_t)pp;
+                    synd.y |= (uint32_t)(pp >> 32);
                     __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
                 }
             }

@@ -93,8 +93,11 @@ ldpc_code *ldpc_code_qc(int mb, int nb, int z, double p, uint64_t seed);
  * spread evenly over checks by a seeded permutation. */
 ldpc_code *ldpc_code_ira(int k_info, int n_hi, int d_hi, int d_lo, uint64_t seed);
 /* New code with the same H whose table order lists checks colour class by
- * colour class (greedy colouring of the check conflict graph), so the serial
- * schedule has few dependency levels. */
+ * colour class, so the serial schedule has few, evenly filled dependency
+ * levels.  Colouring of the check conflict graph (checks sharing a variable
+ * conflict): DSatur order, each check in the least-filled colour its
+ * neighbours leave free, starting from as many colours as the largest
+ * variable degree; deterministic. */
 ldpc_code *ldpc_code_reorder_by_colour(const ldpc_code *code);
 void ldpc_code_destroy(ldpc_code *code);
 
