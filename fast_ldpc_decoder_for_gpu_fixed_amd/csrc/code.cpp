@@ -89,6 +89,11 @@ Plan make_plan(const ldpc_code &c) {
     p.level_beg.assign((size_t)p.n_levels + 1, 0);
     for (int i = 0; i < c.C; i++) p.level_beg[lvl[i] + 1]++;
     for (int l = 0; l < p.n_levels; l++) p.level_beg[l + 1] += p.level_beg[l];
+    p.level_deg.assign((size_t)p.n_levels, -1);
+    for (int i = 0; i < c.C; i++) {
+        int &d = p.level_deg[lvl[i]];
+        d = d < 0 ? c.deg[i] : (d == c.deg[i] ? d : 0);
+    }
     p.slot_of.resize((size_t)c.C);
     p.check_info.resize((size_t)c.C);
     p.check_e0.resize((size_t)c.C);

@@ -39,6 +39,7 @@ struct Plan {
     int n_levels = 0;
     int max_deg = 0;
     std::vector<int> level_beg;         // [n_levels + 1]
+    std::vector<int> level_deg;         // [n_levels] the degree every check of the level has, or 0
     std::vector<int> order;             // level-sorted -> table check index
     std::vector<int> slot_of;           // table check index -> level-sorted
     std::vector<uint32_t> check_info;   // [C] degree | q1 << 8

@@ -94,6 +94,7 @@ struct PoolArgs {
     int pairs;                    // 1: posterior and messages hold 2 frames per 16 bits (P = pairs)
     const uint32_t *rows;         // pair mode, degrees <= 8: [C][8] variable ids per check (unused
                                   // entries 0), and messages as [grid][C][P][8] u16 runs; else null
+    const int *level_deg;         // [n_levels] degree shared by every check of the level, or 0
 };
 // largest posterior block of a frame-queue workgroup; the slot bookkeeping and
 // the learned table (< 4.4 KB) follow it in the same dynamic LDS allocation

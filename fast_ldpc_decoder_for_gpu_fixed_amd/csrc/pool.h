@@ -23,6 +23,7 @@ struct ldpc_pool {
     int stop_rule = 0;
     int pairs = 0;  // posterior and messages as 2-frame pairs (P counts pairs)
     uint32_t *d_rows = nullptr;  // pair mode with degrees <= 8: ids as rows of 8 per check
+    int *d_level_deg = nullptr;  // degree shared by every check of a level, or 0
     int n_cu = 1;   // compute units of the device
     // captured decode step (ldpc_pool_decode_graph) and what it was captured with
     hipStream_t capture = nullptr;

@@ -45,6 +45,7 @@ void pool_free(ldpc_pool *p) {
     (void)hipFree(p->d_table);
     (void)hipFree(p->d_edge_off16);
     (void)hipFree(p->d_rows);
+    (void)hipFree(p->d_level_deg);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -78,6 +79,7 @@ PoolArgs pool_args(const ldpc_pool *p) {
     a.stop_rule = p->stop_rule;
     a.pairs = p->pairs;
     a.rows = p->d_rows;
+    a.level_deg = p->d_level_deg;
     return a;
 }
 
@@ -222,6 +224,7 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     al(&p->d_checks, ck.size());
     al(&p->d_edge_var, (size_t)code->E + ldpc::GATHER_PAD);
     al(&p->d_level_beg, p->plan.level_beg.size());
+    al(&p->d_level_deg, p->plan.level_deg.size());
     al(&p->d_msgs, ((size_t)p->max_grid * msg_u16 * P * wb / code->N + 3) / 4 + ldpc::GATHER_PAD);  // u32 units
     al(&p->d_rows, rows.size());
     al(&p->d_next, 1);
@@ -232,6 +235,9 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     if (e == hipSuccess) e = hipMemset(p->d_edge_var + code->E, 0, (size_t)ldpc::GATHER_PAD * 4);
     if (e == hipSuccess)
         e = hipMemcpy(p->d_level_beg, p->plan.level_beg.data(), p->plan.level_beg.size() * sizeof(int),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_level_deg, p->plan.level_deg.data(), p->plan.level_deg.size() * sizeof(int),
                       hipMemcpyHostToDevice);
     if (e == hipSuccess && !off16.empty())
         e = hipMemcpy(p->d_edge_off16, off16.data(), off16.size() * 4, hipMemcpyHostToDevice);

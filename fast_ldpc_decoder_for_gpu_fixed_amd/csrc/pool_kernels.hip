@@ -664,6 +664,74 @@ __device__ __forceinline__ uint32_t rows_check(uint16_t *post, const uint32_t (&
     for (int j = 0; j < D; j++) post[vi[j]] = half_to_pair(pn[j]);
     return par;
 }
+// One batch of NG rows-layout checks c0, c0 + cs, ... of degree D, all valid,
+// in straight-line code: all loads, then all posterior gathers, then the
+// check updates and stores (checks of one level share no variable, so their
+// order is free).  sel_lo / sel_hi turn the 16-bit message pair in the low /
+// high half of a loaded word into a packed-16 half with the lanes of frames
+// that start now zeroed (the keep mask folded into the unpack).
+template <int D, int NG, bool EQ>
+__device__ __forceinline__ uint32_t rows_batch(const PoolArgs &a, uint16_t *post, uint16_t *mbase, int c0, int cs,
+                                               int P, int lp, int w, uint32_t sel_lo, uint32_t sel_hi, int variant,
+                                               u16x2 lpar) {
+    uint32_t ev[NG][8], mw[NG][4];
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int c = c0 + g * cs;
+        const uint4 *r = reinterpret_cast<const uint4 *>(a.rows + (size_t)c * 8);
+        const uint4 r0 = r[0], r1 = r[1];
+        ev[g][0] = r0.x, ev[g][1] = r0.y, ev[g][2] = r0.z, ev[g][3] = r0.w;
+        ev[g][4] = r1.x, ev[g][5] = r1.y, ev[g][6] = r1.z, ev[g][7] = r1.w;
+        const uint4 m = *reinterpret_cast<const uint4 *>(mbase + ((size_t)c * P + w) * 8);
+        mw[g][0] = m.x, mw[g][1] = m.y, mw[g][2] = m.z, mw[g][3] = m.w;
+    }
+    uint32_t vi[NG][D];
+    u16x2 x[NG][D];
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            vi[g][j] = (ev[g][j] << lp) + (uint32_t)w;
+            x[g][j] = pair_to_half(post[vi[g][j]]);
+        }
+    uint32_t ov = 0;
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        u16x2 m[D], pn[D], mn[D];
+#pragma unroll
+        for (int j = 0; j < D; j++) m[j] = u2(__builtin_amdgcn_perm(0u, mw[g][j >> 1], (j & 1) ? sel_hi : sel_lo));
+        ov |= check_half<D, EQ>(x[g], m, false, variant, false, lpar, pn, mn);
+        uint32_t h[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) h[j] = j < D ? w32(mn[j]) : 0u;
+        uint4 o;
+        o.x = __builtin_amdgcn_perm(h[1], h[0], 0x07050301u);
+        o.y = __builtin_amdgcn_perm(h[3], h[2], 0x07050301u);
+        o.z = __builtin_amdgcn_perm(h[5], h[4], 0x07050301u);
+        o.w = __builtin_amdgcn_perm(h[7], h[6], 0x07050301u);
+        *reinterpret_cast<uint4 *>(mbase + ((size_t)(c0 + g * cs) * P + w) * 8) = o;
+#pragma unroll
+        for (int j = 0; j < D; j++) post[vi[g][j]] = half_to_pair(pn[j]);
+    }
+    return ov;
+}
+
+// One level whose checks all have degree D (the plan's level_deg): a thread
+// takes its checks c0, c0 + cs, ... in batches of NG while NG of them remain,
+// then one at a time, so no lane computes a check it does not own.
+template <int D, int NG, bool EQ>
+__device__ __forceinline__ uint32_t rows_level(const PoolArgs &a, uint16_t *post, uint16_t *mbase, int lb, int le,
+                                               int cs, int P, int lp, int w, uint32_t sel_lo, uint32_t sel_hi,
+                                               int variant, u16x2 lpar) {
+    uint32_t ov = 0;
+    int c0 = lb + ((int)threadIdx.x >> lp);
+    for (; c0 + (NG - 1) * cs < le; c0 += NG * cs)
+        ov |= rows_batch<D, NG, EQ>(a, post, mbase, c0, cs, P, lp, w, sel_lo, sel_hi, variant, lpar);
+    for (; c0 < le; c0 += cs)
+        ov |= rows_batch<D, 1, EQ>(a, post, mbase, c0, cs, P, lp, w, sel_lo, sel_hi, variant, lpar);
+    return ov;
+}
+
 template <bool EQ>
 __device__ __forceinline__ uint32_t rows_check_any(int D, uint16_t *post, const uint32_t (&ev)[8],
                                                    const uint32_t (&mr)[8], uint4 *__restrict__ mp, int lp, int w,
@@ -729,8 +797,40 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
             for (int k = 0; k < 2; k++) rowoff[k] = min(st.it[2 * w + k], a.tits - 1) * a.tdeg;
         uint32_t ov = 0;
         constexpr int NG = DMAXC <= 8 ? 2 : 1;  // checks in flight per thread
+        // message pair -> packed-16 half with fresh frames' lanes zeroed
+        // (the pair in bytes 0-1 or 2-3 of a loaded word)
+        uint32_t sel_lo = 0x010C000Cu;
+        {
+            const uint32_t k16 = s_keep[w];
+            if (!(k16 & 0x00FFu)) sel_lo = (sel_lo & 0xFFFF00FFu) | 0x00000C00u;
+            if (!(k16 & 0xFF00u)) sel_lo = (sel_lo & 0x00FFFFFFu) | 0x0C000000u;
+        }
+        const uint32_t sel_hi = sel_lo + (((sel_lo & 0x0000FF00u) != 0x0C00u ? 0x0200u : 0u) |
+                                          ((sel_lo & 0xFF000000u) != 0x0C000000u ? 0x02000000u : 0u));
         for (int L = 0; L < a.n_levels; L++) {
             const int lb = a.level_beg[L], le = a.level_beg[L + 1];
+            if constexpr (ROWS) {
+                // a level of one degree: straight-line checks (no per-check dispatch)
+                const int ud = a.level_deg ? a.level_deg[L] : 0;
+                if (ud >= 4) {
+                    u16x2 lpar = splat(0u);
+                    if (learned) {
+                        const int col = min(ud, a.tdeg - 1);
+                        lpar = u2(learned_half(a.variant, s_tab[rowoff[0] + col]) |
+                                  (learned_half(a.variant, s_tab[rowoff[1] + col]) << 16));
+                    }
+                    switch (ud) {
+#define LDPC_RLEVEL(d)                                                                                          \
+    case d:                                                                                                     \
+        ov |= rows_level<d, NG, EQ>(a, post, mbase, lb, le, cstride, P, lp, w, sel_lo, sel_hi, a.variant, lpar); \
+        break;
+                        LDPC_RLEVEL(4) LDPC_RLEVEL(5) LDPC_RLEVEL(6) LDPC_RLEVEL(7) LDPC_RLEVEL(8)
+#undef LDPC_RLEVEL
+                    }
+                    __syncthreads();
+                    continue;
+                }
+            }
             for (int c0 = lb + (tid >> lp); c0 < le; c0 += NG * cstride) {
                 uint2 ci[NG];
                 uint32_t ev[NG][DMAXC], mr[NG][DMAXC];
