@@ -101,6 +101,15 @@ class Code:
         check(load().ldpc_code_export(self._h, ptr(deg, C.c_int), ptr(pos, C.c_uint32)), "ldpc_code_export")
         return deg, pos
 
+    def levels(self, granule: int = 1):
+        """Level of every check in the decoders' parallel schedule (granule > 1:
+        level sizes in multiples of granule where the dependencies allow)."""
+        out = np.empty(self.n_checks, np.int32)
+        n = load().ldpc_code_levels(self._h, granule, ptr(out, C.c_int))
+        if n < 0:
+            check(n, "ldpc_code_levels")
+        return out
+
 
 def code_3_6(n: int = 4000, seed: int = 1, colour: bool = True) -> Code:
     """(dv=3, dc=6) code from a seeded socket permutation with 4-cycles rejected;

@@ -49,6 +49,7 @@ SIGNATURES = {
     "ldpc_code_destroy": (None, [C.c_void_p]),
     "ldpc_code_info": (C.c_int, [C.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p]),
     "ldpc_code_export": (C.c_int, [C.c_void_p, _c_int_p, _u32_p]),
+    "ldpc_code_levels": (C.c_int, [C.c_void_p, C.c_int, _c_int_p]),
     "ldpc_decoder_create": (C.c_void_p, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(Tuning)]),
     "ldpc_decoder_initialize": (C.c_int, [C.c_void_p]),
     "ldpc_decoder_destroy": (None, [C.c_void_p]),

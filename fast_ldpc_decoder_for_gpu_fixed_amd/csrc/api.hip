@@ -275,6 +275,15 @@ int ldpc_code_info(const ldpc_code *code, int *n_vars, int *n_checks, int *n_edg
     return LDPC_OK;
 }
 
+int ldpc_code_levels(const ldpc_code *code, int granule, int *level_of) {
+    if (!code) return fail(LDPC_ERR_ARG, "null code");
+    const ldpc::Plan p = ldpc::make_plan(*code, granule);
+    if (level_of)
+        for (int l = 0; l < p.n_levels; l++)
+            for (int s = p.level_beg[l]; s < p.level_beg[l + 1]; s++) level_of[p.order[s]] = l;
+    return p.n_levels;
+}
+
 int ldpc_code_export(const ldpc_code *code, int *deg, uint32_t *pos) {
     if (!code) return fail(LDPC_ERR_ARG, "null code");
     if (deg) std::copy(code->deg.begin(), code->deg.end(), deg);
@@ -302,7 +311,6 @@ ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
     if (tuning) tu = *tuning;
     ldpc_decoder *d = new ldpc_decoder();
     d->code = *code;
-    d->plan = ldpc::make_plan(d->code);
     d->W = nb_frames_words;
     d->variant = variant;
     d->early_term = tu.early_term >= 0 ? (tu.early_term != 0) : (variant == LDPC_2NMS);
@@ -330,6 +338,8 @@ ldpc_decoder *ldpc_decoder_create(const ldpc_code *code, size_t nb_frames_words,
     }
     d->P = P;
     while ((1 << d->log2P) < P) d->log2P++;
+    // a level's (check, word) items fill 64-lane waves: sizes in multiples of 64 / P checks
+    d->plan = ldpc::make_plan(d->code, 64 / P);
     int T = tu.threads > 0 ? tu.threads : 256;
     if (T % 64 || T % P || T < 64 || T > 1024) {
         free_decoder(d);

@@ -74,10 +74,73 @@ int dependency_levels(const ldpc_code &c, std::vector<int> *level_of) {
     return n_levels;
 }
 
-Plan make_plan(const ldpc_code &c) {
+// Level sizes rounded to whole granules of checks, the table's order kept.
+// Two checks that share a variable keep their table order (the earlier one
+// sits on a lower level), so every assignment built here decodes bit for bit
+// like the serial schedule; only checks with slack move.  List scheduling
+// over the n_levels of `dependency_levels`: a check is ready once every
+// earlier check sharing a variable with it sits on a lower level; its latest
+// level (alap) is one below the earliest latest level of the later checks
+// sharing a variable with it.  Level l takes the ready checks whose latest
+// level is l, then ready checks by (alap, table index) up to the multiple of
+// `granule` nearest to the remaining checks per remaining level (all that is
+// ready on the last level).  Placing every check between its ready level and
+// its alap keeps the number of levels.
+static void balance_levels(const ldpc_code &c, int n_levels, int granule, std::vector<int> &lvl) {
+    const int C = c.C;
+    std::vector<std::vector<int>> succ((size_t)C);
+    std::vector<int> npred((size_t)C, 0), last((size_t)c.N, -1);
+    for (int i = 0; i < C; i++) {
+        std::vector<int> pr;
+        for (size_t e = c.eoff[i]; e < c.eoff[i + 1]; e++) {
+            const int q = last[c.pos[e]];
+            if (q >= 0) pr.push_back(q);
+            last[c.pos[e]] = i;
+        }
+        std::sort(pr.begin(), pr.end());
+        pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+        npred[i] = (int)pr.size();
+        for (int q : pr) succ[(size_t)q].push_back(i);
+    }
+    std::vector<int> alap((size_t)C, n_levels - 1);
+    for (int i = C - 1; i >= 0; i--)
+        for (int s : succ[(size_t)i]) alap[i] = std::min(alap[i], alap[s] - 1);
+    std::vector<int> ready;
+    for (int i = 0; i < C; i++)
+        if (npred[i] == 0) ready.push_back(i);
+    int left = C;
+    for (int l = 0; l < n_levels; l++) {
+        std::sort(ready.begin(), ready.end(), [&](int a, int b) {
+            return alap[a] != alap[b] ? alap[a] < alap[b] : a < b;
+        });
+        const int r = (int)ready.size();
+        int forced = 0;
+        while (forced < r && alap[ready[forced]] <= l) forced++;
+        int n = r;
+        if (l < n_levels - 1) {
+            const double target = (double)left / (n_levels - l);
+            const int lo = (forced + granule - 1) / granule * granule;
+            if (lo <= r) {
+                n = lo;
+                for (int m = lo; m <= r; m += granule)
+                    if (std::abs(m - target) < std::abs(n - target)) n = m;
+            }
+        }
+        std::vector<int> next(ready.begin() + n, ready.end());
+        for (int k = 0; k < n; k++) lvl[ready[k]] = l;
+        for (int k = 0; k < n; k++)
+            for (int s : succ[(size_t)ready[k]])
+                if (--npred[s] == 0) next.push_back(s);
+        left -= n;
+        ready.swap(next);
+    }
+}
+
+Plan make_plan(const ldpc_code &c, int granule) {
     Plan p;
     std::vector<int> lvl;
     p.n_levels = dependency_levels(c, &lvl);
+    if (granule > 1 && p.n_levels > 1) balance_levels(c, p.n_levels, granule, lvl);
     p.max_deg = c.max_deg;
     p.order.resize((size_t)c.C);
     std::iota(p.order.begin(), p.order.end(), 0);

@@ -46,7 +46,12 @@ struct Plan {
     std::vector<uint32_t> check_e0;     // [C] first level-order edge
     std::vector<uint32_t> edge_var;     // [E] level-order
 };
-Plan make_plan(const ldpc_code &c);
+// granule <= 1: the levels of dependency_levels.  granule > 1: the same number
+// of levels, with checks that have slack moved so that level sizes are
+// multiples of `granule` where the dependencies allow (a kernel that maps
+// (check, word) items of a level onto 64-lane waves passes 64 / words per
+// block, so that the waves at a level's end are full); decisions are the same.
+Plan make_plan(const ldpc_code &c, int granule = 1);
 
 // generators (deterministic)
 std::string gen_regular(ldpc_code &out, int n, int dv, int dc, uint64_t seed);

@@ -168,7 +168,9 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     }
     ldpc_pool *p = new ldpc_pool();
     p->code = *code;
-    p->plan = ldpc::make_plan(p->code);
+    // word blocks: level sizes in multiples of 64 / P checks (full waves); pair mode keeps the
+    // dependency levels, whose single-degree levels take the straight-line path
+    p->plan = ldpc::make_plan(p->code, pairs ? 1 : 64 / P);
     p->variant = variant;
     p->max_iters = max_iterations;
     p->stop_rule = stop_rule;

@@ -107,6 +107,17 @@ int ldpc_code_info(const ldpc_code *code, int *n_vars, int *n_checks, int *n_edg
                    int *max_deg, int *n_levels, int *q1_split);
 /* Copy out the table (deg[n_checks], pos[n_edges]). */
 int ldpc_code_export(const ldpc_code *code, int *deg, uint32_t *pos);
+/* The level of every check in the decoders' parallel schedule of the table
+ * (level_of[n_checks], may be NULL); returns the number of levels, or a
+ * negative status.  granule <= 1: dependency levels (a check's level is one
+ * above the highest level of the earlier checks sharing a variable with it).
+ * granule > 1: the same number of levels, checks with slack moved so that
+ * level sizes are multiples of granule where the dependencies allow; the word
+ * decoders use 64 / words_per_block (full 64-lane waves).  Checks sharing a
+ * variable keep their table order either way, so decoding is unchanged.  No
+ * reference counterpart: the reference runs the table serially per word
+ * (src/decoder_oms/CUDA_OMS_SIMD.cu:148-191). */
+int ldpc_code_levels(const ldpc_code *code, int granule, int *level_of);
 
 /* ---- decoder objects (replaces class CGPUDecoder,
  * src/decoder_template/CGPUDecoder.h:19-35, and its five subclasses) ------ */

@@ -135,6 +135,60 @@ def test_balanced_colouring():
         assert c.n_levels >= np.bincount(p_).max()
 
 
+def _table_levels(deg, pos, n):
+    """Dependency levels of the serial table order (numpy restatement)."""
+    last = np.full(n, -1)
+    out, e = [], 0
+    for d in deg:
+        vs = pos[e:e + d]
+        e += d
+        out.append(int(last[vs].max()) + 1)
+        last[vs] = out[-1]
+    return np.array(out)
+
+
+def _keeps_table_order(deg, pos, lv):
+    """Every variable's checks, in table order, sit on strictly increasing levels."""
+    off = np.concatenate([[0], np.cumsum(deg)])
+    seen = {}
+    for c in range(len(deg)):
+        for v in pos[off[c]:off[c + 1]]:
+            if v in seen and not lv[seen[v]] < lv[c]:
+                return False
+            seen[v] = c
+    return True
+
+
+def _lane_waste(sizes, words):
+    return int(sum(-(-n * words // 64) * 64 - n * words for n in sizes))
+
+
+def test_wave_balanced_levels():
+    """Levels with sizes in whole waves: the same number of levels, checks that
+    share a variable still in table order (so decisions do not change), and for
+    the (3,6) N=4000 benchmark code with 2-word blocks the minimum of partly
+    filled wave lanes (4000 items per iteration = 62.5 waves: 32 lanes)."""
+    code = L.code_3_6(4000)
+    deg, pos = code.export()
+    asap = _table_levels(deg, pos, 4000)
+    np.testing.assert_array_equal(code.levels(1), asap)
+    assert list(np.bincount(asap)) == [315, 311, 307, 312, 302, 310, 143]
+    lv = code.levels(32)
+    assert lv.max() + 1 == asap.max() + 1
+    assert _keeps_table_order(deg, pos, lv)
+    sizes = np.bincount(lv)
+    assert list(sizes) == [288, 320, 288, 320, 288, 256, 240]
+    assert _lane_waste(np.bincount(asap), 2) == 160 and _lane_waste(sizes, 2) == 32
+    assert not _keeps_table_order(deg, pos, np.zeros_like(lv))  # the check detects a violation
+    for c, g in ((L.Code.regular(240, 3, 6, 5), 8), (L.Code.qc(12, 24, 81, 0.35, seed=1), 32),
+                 (L.Code.qc(4, 24, 96, 0.35, seed=2), 16), (L.Code.ira(3240, 1296, 8, 3, 11).reorder_by_colour(), 64)):
+        d, p_ = c.export()
+        a = _table_levels(d, p_, c.n_vars)
+        b = c.levels(g)
+        assert b.max() == a.max() and _keeps_table_order(d, p_, b)
+        assert _lane_waste(np.bincount(b), 64 // g) <= _lane_waste(np.bincount(a), 64 // g)
+
+
 @pytest.mark.skipif(not (REFERENCE / "src/matrix").exists(), reason="reference tree not present")
 @pytest.mark.parametrize("table,n,expect", [
     ("4000x2000", 4000, dict(C=2000, E=12000, levels=26, colours=7, maxdeg=6)),

@@ -206,6 +206,33 @@ def test_dag_respecting_reorder_is_bit_identical(variant):
     assert not np.array_equal(a.post, c3.post)
 
 
+def test_wave_balanced_levels_decode_like_the_table():
+    """The decoders' wave-balanced schedule (checks with slack moved between
+    levels) run in level order by the oracle gives the serial table's posterior
+    and messages."""
+    import fast_ldpc_decoder_for_gpu_fixed_amd as L
+    code = L.Code.regular(240, 3, 6, 5)
+    deg, pos = code.export()
+    asap = np.array(levels_py(list(deg), list(pos), 240))
+    lv = code.levels(8)
+    assert lv.max() == asap.max() and not np.array_equal(lv, asap)  # some checks moved
+    order = sorted(range(len(deg)), key=lambda c: (lv[c], c))
+    deg2, pos2 = reorder(deg, pos, order)
+    rng = np.random.default_rng(4)
+    W = 8
+    llr = rng.integers(-40, 30, size=4 * W * 240).astype(np.int8)
+    a = O.OracleDecoder(deg, pos, 240, W, O.OMS, early_term=0)
+    b = O.OracleDecoder(deg2, pos2, 240, W, O.OMS, early_term=0)
+    a.load(llr)
+    b.load(llr)
+    a.iterate(5)
+    b.iterate(5)
+    np.testing.assert_array_equal(a.post, b.post)
+    ma, mb = msg_by_check(a, deg), msg_by_check(b, deg2)
+    for i, c in enumerate(order):
+        np.testing.assert_array_equal(ma[c], mb[i])
+
+
 # ---------------------------------------------------------------------------
 # layout: emulate the reference interleaver's index arithmetic
 # ---------------------------------------------------------------------------
