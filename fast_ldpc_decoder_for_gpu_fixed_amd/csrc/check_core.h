@@ -228,53 +228,78 @@ __device__ __forceinline__ void store_msgs(uint32_t *__restrict__ mp, const uint
 // items c_first, c_first + cstride, ... < le of word w.  The check's edges
 // start at c*D, its LDS byte offsets (v * P * 4) come packed two per dword
 // (edge_off16), and the index and message loads of G items are issued
-// together before any of them is computed.  FIRST: no message read (the
-// decode's first iteration).  keep: byte mask of the message lanes read (a
-// lane whose frame starts now reads zero messages; folded into the unpack).  VK >= 0: the
-// variant known at compile time (its magnitude map is then straight-line code
-// with no per-item branch), -1: read from `variant`.  Returns the OR of the
-// items' parity bits (check_core's layout).
+// together before any of them is computed: full batches of G while G items
+// remain, then single items, so no load or update is predicated inside a
+// batch.  FIRST: no message read (the decode's first iteration).  keep: byte
+// mask of the message lanes read (a lane whose frame starts now reads zero
+// messages; folded into the unpack).  VK >= 0: the variant known at compile
+// time (its magnitude map is then straight-line code with no per-item
+// branch), -1: read from `variant`.  Returns the OR of the items' parity
+// bits (check_core's layout).
+// Offsets are 32-bit byte offsets from the (uniform) table and message
+// bases, so the loads and stores take the scalar-base + 32-bit vector-offset
+// form and no 64-bit multiply is formed per item.
 // ---------------------------------------------------------------------------
+template <int D, int G, bool EQ, bool FIRST, int VK>
+__device__ __forceinline__ uint2 uniform_batch(const char *obase, char *mbytes, char *pbase, uint32_t ob,
+                                               uint32_t mb, uint32_t ostep, uint32_t mstep, int variant,
+                                               const MsgUnpack &mu, u16x2 lpe, u16x2 lpo) {
+    constexpr int H = (D + 1) / 2;
+    uint32_t off[G][H];
+    uint32_t mw[G][D];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const uint32_t *op = reinterpret_cast<const uint32_t *>(obase + (ob + g * ostep));
+#pragma unroll
+        for (int h = 0; h < H; h++) off[g][h] = op[h];
+        if (!FIRST) load_msgs<D>(reinterpret_cast<const uint32_t *>(mbytes + (mb + g * mstep)), mw[g]);
+    }
+    uint2 ov = make_uint2(0u, 0u);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        uint32_t *vp[D];
+        uint32_t pw[D], pn[D], mn[D];
+#pragma unroll
+        for (int j = 0; j < D; j++)
+            vp[j] = reinterpret_cast<uint32_t *>(pbase + ((j & 1) ? (off[g][j >> 1] >> 16) : (off[g][j >> 1] & 0xFFFFu)));
+#pragma unroll
+        for (int j = 0; j < D; j++) pw[j] = *vp[j];
+        const uint2 pr = check_core<D, EQ>(pw, mw[g], FIRST, VK >= 0 ? VK : variant, false, lpe, lpo, pn, mn, mu);
+        ov.x |= pr.x;
+        ov.y |= pr.y;
+        store_msgs<D>(reinterpret_cast<uint32_t *>(mbytes + (mb + g * mstep)), mn);
+#pragma unroll
+        for (int j = 0; j < D; j++) *vp[j] = pn[j];
+    }
+    return ov;
+}
+
 template <int D, int G, bool EQ, bool FIRST, int VK = -1>
 __device__ __forceinline__ uint2 uniform_items(const uint32_t *__restrict__ edge_off16, int P, int variant,
                                                uint32_t *post, uint32_t *mbase, int c_first, int le, int cstride,
                                                int w, uint32_t keep, u16x2 lpe, u16x2 lpo) {
     constexpr int H = (D + 1) / 2;
-    if constexpr (VK >= 0) variant = VK;
     char *const pbase = reinterpret_cast<char *>(post) + 4 * w;
+    const char *const obase = reinterpret_cast<const char *>(edge_off16);
+    char *const mbytes = reinterpret_cast<char *>(mbase);
     const MsgUnpack mu = msg_unpack(keep);
+    // check indices and P fit 24 bits: v_mul_u32_u24, not v_mul_lo_u32
+    const uint32_t ostep = __umul24(cstride, 4 * H), mstep = __umul24(__umul24(cstride, P), 4 * D);
+    uint32_t ob = __umul24(c_first, 4 * H), mb = __umul24(__umul24(c_first, P) + w, 4 * D);
     uint2 ov = make_uint2(0u, 0u);
-    for (int c0 = c_first; c0 < le; c0 += G * cstride) {
-        uint32_t off[G][H];
-        uint32_t mw[G][D];
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-            const int c = c0 + g * cstride;
-            if (c < le) {
-#pragma unroll
-                for (int h = 0; h < H; h++) off[g][h] = edge_off16[(size_t)c * H + h];
-                if (!FIRST) load_msgs<D>(mbase + ((size_t)c * P + w) * D, mw[g]);
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-            const int c = c0 + g * cstride;
-            if (c < le) {
-                uint32_t *vp[D];
-                uint32_t pw[D], pn[D], mn[D];
-#pragma unroll
-                for (int j = 0; j < D; j++)
-                    vp[j] = reinterpret_cast<uint32_t *>(
-                        pbase + ((j & 1) ? (off[g][j >> 1] >> 16) : (off[g][j >> 1] & 0xFFFFu)));
-#pragma unroll
-                for (int j = 0; j < D; j++) pw[j] = *vp[j];
-                const uint2 pr = check_core<D, EQ>(pw, mw[g], FIRST, variant, false, lpe, lpo, pn, mn, mu);
-                ov.x |= pr.x;
-                ov.y |= pr.y;
-                store_msgs<D>(mbase + ((size_t)c * P + w) * D, mn);
-#pragma unroll
-                for (int j = 0; j < D; j++) *vp[j] = pn[j];
-            }
+    int c = c_first;
+    for (; c + (G - 1) * cstride < le; c += G * cstride, ob += G * ostep, mb += G * mstep) {
+        const uint2 pr = uniform_batch<D, G, EQ, FIRST, VK>(obase, mbytes, pbase, ob, mb, ostep, mstep, variant, mu,
+                                                            lpe, lpo);
+        ov.x |= pr.x;
+        ov.y |= pr.y;
+    }
+    if constexpr (G > 1) {
+        for (; c < le; c += cstride, ob += ostep, mb += mstep) {
+            const uint2 pr = uniform_batch<D, 1, EQ, FIRST, VK>(obase, mbytes, pbase, ob, mb, ostep, mstep, variant,
+                                                                mu, lpe, lpo);
+            ov.x |= pr.x;
+            ov.y |= pr.y;
         }
     }
     return ov;

@@ -214,8 +214,9 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 16 ? 3 : 1))) void k_deco
         __syncthreads();
         const bool act = s_active[w] != 0;
         uint32_t ov = 0;
+        int lb = a.level_beg[0], le = a.level_beg[1];
         for (int L = 0; L < a.n_levels; L++) {
-            const int lb = a.level_beg[L], le = a.level_beg[L + 1];
+            const int ln = a.level_beg[min(L + 2, a.n_levels)];  // the next level's end, loaded early
             if constexpr (UD > 0) {
                 if (act) {
                     const int lpar = a.ltable ? (int)s_lrow[UD] : 0;
@@ -236,6 +237,8 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 16 ? 3 : 1))) void k_deco
                 }
             }
             __syncthreads();
+            lb = le;
+            le = ln;
         }
         if (act) ran++;
         if (a.early_term && !first) {

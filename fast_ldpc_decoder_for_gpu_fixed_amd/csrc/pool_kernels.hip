@@ -67,13 +67,15 @@ __device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_
             // SB checks per thread at a time: their offsets are loaded together
             constexpr int SB = 4;
             const char *pbase = reinterpret_cast<const char *>(post);
+            const char *obase = reinterpret_cast<const char *>(a.edge_off16);
             for (int c0 = tid; c0 < a.C; c0 += SB * T) {
                 uint32_t off[SB][H];
 #pragma unroll
                 for (int b = 0; b < SB; b++) {
                     const int c = min(c0 + b * T, a.C - 1);
+                    const uint32_t *op = reinterpret_cast<const uint32_t *>(obase + __umul24(c, 4 * H));
 #pragma unroll
-                    for (int h = 0; h < H; h++) off[b][h] = a.edge_off16[(size_t)c * H + h];
+                    for (int h = 0; h < H; h++) off[b][h] = op[h];
                 }
 #pragma unroll
                 for (int b = 0; b < SB; b++) {
@@ -273,21 +275,23 @@ __device__ __forceinline__ void pool_round(const PoolArgs &a, const SlotState &s
 }
 
 // every thread: decisions of the frames leaving their slots, LLRs of the
-// frames entering them (byte lane k of word wl of each variable; B = bytes
-// per lane group: 4 for words, 2 for pairs).  Per slot and thread, IO_BATCH
-// LLR dwords are loaded first; then the leaving frame's decisions are read
-// from the lanes and stored back to back (the stores leave no register to
-// wait on); then the loaded LLRs are written into the same lanes.
+// frames entering them (byte lane k of element wl of each variable; B = bytes
+// per element: 4 for words, 2 for pairs).  A thread takes dword q of a frame
+// (variables 4q..4q+3).  Per slot and thread, IO_BATCH LLR dwords are loaded
+// first; then per dword the leaving frame's decisions (the 4 elements read
+// whole, byte k of each gathered into one dword by two v_perm and an OR, the
+// positive-byte test giving the 4 decisions at once) and the entering
+// frame's LLR bytes written into the same lanes.
+// The variable order within a dword is rotated by (lane / 8) % 4, so that
+// the lanes of a 32-lane LDS access fall on distinct banks; the gathered and
+// loaded dwords are rotated to match.
 constexpr int IO_BATCH = 8;
 template <int B>
 __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, uint8_t *pb, int S, int P) {
     int tid = threadIdx.x;
     __asm__ volatile("" : "+v"(tid));  // as in pool_round: no per-thread addresses hoisted
     const int T = blockDim.x, N = a.N;
-    const size_t stride = (size_t)P * B;
-    // A thread's dword q covers variables 4q..4q+3, whose lane bytes are 4P
-    // words apart, so for a fixed byte b the lanes of a wave fall on few LDS
-    // banks; rotating the byte order by (lane / 8) % 4 spreads them.
+    const uint32_t stride = (uint32_t)P * B;  // bytes per variable
     const int rot = (tid >> 3) & 3;
     for (int s = 0; s < S; s++) {
         const int io = st.io[s];
@@ -298,6 +302,22 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
             uint32_t *dst = reinterpret_cast<uint32_t *>(a.out_hard + wo);
             const uint32_t *src = reinterpret_cast<const uint32_t *>(a.in_frames + lo);
             const int Q = N >> 2;
+            // element b is variable 4q + ((b + rot) & 3): its byte offset in
+            // the group of 4 variables, and the gather selectors that put its
+            // byte k at byte (b + rot) & 3 of the decision dword (v_perm
+            // selector bytes: 0-3 pick src1, 4-7 src0, 0x0C gives zero)
+            uint32_t eoff[4], sel01 = 0x0C0C0C0Cu, sel23 = 0x0C0C0C0Cu;
+#pragma unroll
+            for (int b = 0; b < 4; b++) eoff[b] = (uint32_t)((b + rot) & 3) * stride;
+            {
+                const uint32_t p0 = (uint32_t)((0 + rot) & 3), p1 = (uint32_t)((1 + rot) & 3);
+                const uint32_t p2 = (uint32_t)((2 + rot) & 3), p3 = (uint32_t)((3 + rot) & 3);
+                sel01 = (sel01 & ~(0xFFu << (8 * p0))) | ((uint32_t)k << (8 * p0));
+                sel01 = (sel01 & ~(0xFFu << (8 * p1))) | ((uint32_t)(4 + k) << (8 * p1));
+                sel23 = (sel23 & ~(0xFFu << (8 * p2))) | ((uint32_t)k << (8 * p2));
+                sel23 = (sel23 & ~(0xFFu << (8 * p3))) | ((uint32_t)(4 + k) << (8 * p3));
+            }
+            const uint32_t lane0 = (uint32_t)wl * B;  // element of word/pair wl within a variable
             for (int q0 = tid; q0 < Q; q0 += IO_BATCH * T) {
                 uint32_t y[IO_BATCH];
                 if (io & 2) {
@@ -307,40 +327,35 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
                         y[r] = q < Q ? src[q] : 0u;
                     }
                 }
-                if (io & 1) {
-                    uint32_t x[IO_BATCH];
 #pragma unroll
-                    for (int r = 0; r < IO_BATCH; r++) {
-                        int q = q0 + r * T;
-                        __asm__ volatile("" : "+v"(q));
-                        x[r] = 0;
-                        if (q < Q) {
-                            const uint8_t *lane = pb + (size_t)(4 * q) * stride + (size_t)wl * B + k;
+                for (int r = 0; r < IO_BATCH; r++) {
+                    int q = q0 + r * T;
+                    __asm__ volatile("" : "+v"(q));
+                    if (q < Q) {
+                        // the 4 elements' addresses serve the decision reads and
+                        // the LLR writes (a lane's reads come before its writes)
+                        uint8_t *g[4];
+#pragma unroll
+                        for (int b = 0; b < 4; b++) g[b] = pb + (__umul24(4 * q, stride) + lane0 + eoff[b]);
+                        if (io & 1) {
+                            uint32_t e[4];
 #pragma unroll
                             for (int b = 0; b < 4; b++) {
-                                const int bb = (b + rot) & 3;
-                                x[r] |= (uint32_t)((int8_t)lane[bb * stride] > 0) << (8 * bb);
+                                if constexpr (B == 4)
+                                    e[b] = *reinterpret_cast<const uint32_t *>(g[b]);
+                                else
+                                    e[b] = *reinterpret_cast<const uint16_t *>(g[b]);
                             }
+                            // bytes of variables 4q..4q+3 in order, then per byte: 1 if > 0
+                            const uint32_t v = __builtin_amdgcn_perm(e[1], e[0], sel01) |
+                                               __builtin_amdgcn_perm(e[3], e[2], sel23);
+                            dst[q] = ((((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) & ~v) >> 7) & 0x01010101u;
                         }
-                    }
+                        if (io & 2) {
+                            // z byte b = LLR of variable 4q + ((b + rot) & 3)
+                            const uint32_t z = __builtin_amdgcn_alignbyte(y[r], y[r], (uint32_t)rot);
 #pragma unroll
-                    for (int r = 0; r < IO_BATCH; r++) {
-                        const int q = q0 + r * T;
-                        if (q < Q) dst[q] = x[r];
-                    }
-                }
-                if (io & 2) {
-#pragma unroll
-                    for (int r = 0; r < IO_BATCH; r++) {
-                        int q = q0 + r * T;
-                        __asm__ volatile("" : "+v"(q));
-                        if (q < Q) {
-                            uint8_t *lane = pb + (size_t)(4 * q) * stride + (size_t)wl * B + k;
-#pragma unroll
-                            for (int b = 0; b < 4; b++) {
-                                const int bb = (b + rot) & 3;
-                                lane[bb * stride] = (uint8_t)(y[r] >> (8 * bb));
-                            }
+                            for (int b = 0; b < 4; b++) g[b][k] = (uint8_t)(z >> (8 * b));
                         }
                     }
                 }
@@ -419,13 +434,17 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 10 ? 4 : UD == 0 && DMAXC
                 lpo = u2(learned_half(a.variant, s_tab[rowoff[1] + col]) |
                          (learned_half(a.variant, s_tab[rowoff[3] + col]) << 16));
             }
+            int lb = a.level_beg[0], le = a.level_beg[1];
             for (int L = 0; L < a.n_levels; L++) {
+                // the next level's end, loaded while this level runs
+                const int ln = a.level_beg[min(L + 2, a.n_levels)];
                 const uint2 pr = uniform_items<UD, G, EQ, false, VK>(a.edge_off16, P, a.variant, post, mbase,
-                                                                 a.level_beg[L] + (tid >> lp), a.level_beg[L + 1],
-                                                                 cstride, w, keep, lpe, lpo);
+                                                                 lb + (tid >> lp), le, cstride, w, keep, lpe, lpo);
                 ove |= pr.x;
                 ovo |= pr.y;
                 __syncthreads();
+                lb = le;
+                le = ln;
             }
         } else {
             for (int L = 0; L < a.n_levels; L++) {
