@@ -86,7 +86,7 @@ __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 
     constexpr bool KEEP = D <= 12;  // keep |t| and -t live across the two passes
     u16x2 t[D], ng[KEEP ? D : 1], ab[KEEP ? D : 1];
     u16x2 m1 = splat(0x7F00u), m2 = m1;
-    u16x2 sg = splat(0u);
+    u16x2 sg = splat(0u), yp = splat(0u);
     const u16x2 zero = splat(0u);
 #pragma unroll
     for (int j = 0; j < D; j++) {
@@ -96,7 +96,12 @@ __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 
         t[j] = v;
         const u16x2 y = sub_sat_s(zero, v);  // -t, top bit = (t > 0)
         const u16x2 a = umin(v, y);          // |t| as uint
-        sg ^= y;
+        // parity of the positives: two edges per 3-input XOR (v_bitop3 0x96)
+        if (j & 1)
+            sg = u2(__builtin_amdgcn_bitop3_b32(w32(sg), w32(yp), w32(y), 0x96));
+        else if (j == D - 1)
+            sg ^= y;
+        yp = y;
         if (j == 0) {
             // min2 = min(127, max(a, 127)) = 127 after the first edge
             m1 = umin(m1, a);

@@ -39,17 +39,24 @@ __device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__re
     return pr;
 }
 
-// per byte of a posterior word: bit 7 set if the byte is > 0 (decision 1).
-// The low 7 bits plus 0x7F carry into bit 7 exactly when they are nonzero
-// (no carry leaves the byte); & ~x clears negative bytes.  Bits 0..6 of the
-// result are junk: XOR-accumulate, then keep 0x80808080 once per check.
-__device__ __forceinline__ uint32_t positive_msb(uint32_t x) {
-    return ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) & ~x;
+// Decision parity accumulation: acc ^ (per byte of the posterior word x, bit
+// 7 set if the byte is > 0, i.e. decision 1).  The low 7 bits plus 0x7F carry
+// into bit 7 exactly when they are nonzero (no carry leaves the byte), and
+// & ~x clears negative bytes; the AND-NOT and the XOR into the accumulator
+// are one v_bitop3 (truth table 0x9A: (u & ~x) ^ acc, table index
+// u << 2 | x << 1 | acc).  Bits 0..6 of each byte are junk: keep 0x80808080
+// once per check.
+__device__ __forceinline__ uint32_t xor_positive(uint32_t acc, uint32_t x) {
+    const uint32_t u = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+    return __builtin_amdgcn_bitop3_b32(u, x, acc, 0x9A);
 }
 // the same on two words at once: no carry crosses a byte, so one 64-bit add
 // (v_lshl_add_u64) does the work of two 32-bit adds
-__device__ __forceinline__ uint64_t positive_msb64(uint64_t x) {
-    return ((x & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) & ~x;
+__device__ __forceinline__ uint64_t xor_positive64(uint64_t acc, uint64_t x) {
+    const uint64_t u = (x & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full;
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)u, (uint32_t)x, (uint32_t)acc, 0x9A);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(u >> 32), (uint32_t)(x >> 32), (uint32_t)(acc >> 32), 0x9A);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // syndrome of the decisions over every check, per byte (bit 7 of each byte
@@ -83,7 +90,7 @@ __device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_
 #pragma unroll
                     for (int j = 0; j < UD; j++) {
                         const uint32_t o = (j & 1) ? (off[b][j >> 1] >> 16) : (off[b][j >> 1] & 0xFFFFu);
-                        pp ^= positive_msb64(*reinterpret_cast<const uint64_t *>(pbase + o));
+                        pp = xor_positive64(pp, *reinterpret_cast<const uint64_t *>(pbase + o));
                     }
                     // a clamped index repeats check C-1: OR-ing its parity twice is harmless
                     synd.x |= (uint32_t)pp;
@@ -101,7 +108,7 @@ __device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_
 #pragma unroll
                 for (int j = 0; j < UD; j++) {
                     const uint32_t o = (j & 1) ? (off[j >> 1] >> 16) : (off[j >> 1] & 0xFFFFu);
-                    par ^= positive_msb(*reinterpret_cast<const uint32_t *>(pbase + o));
+                    par = xor_positive(par, *reinterpret_cast<const uint32_t *>(pbase + o));
                 }
                 synd.x |= par;
             }
@@ -499,7 +506,7 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 10 ? 4 : UD == 0 && DMAXC
                     uint32_t par = 0;
 #pragma unroll
                     for (int j = 0; j < DMAXC; j++)
-                        if (j < D) par ^= positive_msb(post[(ev[g][j] << lp) + (uint32_t)w]);
+                        if (j < D) par = xor_positive(par, post[(ev[g][j] << lp) + (uint32_t)w]);
                     found |= par & 0x80808080u;
                     __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
                 }
@@ -902,7 +909,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                     uint32_t par = 0;
 #pragma unroll
                     for (int j = 0; j < DMAXC; j++)
-                        if (j < D) par ^= positive_msb(post[(ev[g][j] << lp) + (uint32_t)w]);
+                        if (j < D) par = xor_positive(par, post[(ev[g][j] << lp) + (uint32_t)w]);
                     found |= par & 0x8080u;
                     __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
                 }
