@@ -569,7 +569,9 @@ std::string reorder_by_colour(const ldpc_code &in, ldpc_code &out) {
     int k = iterated_greedy(nbr, colour, (int)size.size(), 64);
     Rng rng(0xC0104EDull ^ ((uint64_t)in.C << 20) ^ (uint64_t)in.E);
     const long budget = 20000 + 4L * in.C;
-    while (k > (int)k0) {
+    // the tabu stage keeps two C x k tables (12 bytes per entry): tables
+    // beyond 2^22 entries (~50 MB) keep the iterated-greedy colouring
+    while (k > (int)k0 && (size_t)in.C * (size_t)k <= ((size_t)1 << 22)) {
         std::vector<int> cnt((size_t)k, 0);
         for (int c : colour) cnt[c]++;
         const int drop = (int)(std::min_element(cnt.begin(), cnt.end()) - cnt.begin());
