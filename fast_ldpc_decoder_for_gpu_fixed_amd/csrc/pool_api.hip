@@ -216,7 +216,7 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     // runs of 8 u16 per check and pair (16-byte aligned, addressed by the check)
     std::vector<uint32_t> rows;
     if (p->pairs && code->max_deg <= 8) {
-        rows.assign((size_t)code->C * 8, 0u);
+        rows.assign((size_t)code->C * 8, (uint32_t)code->N);  // unused entries: the padding variable
         for (int c = 0; c < code->C; c++) {
             const int D = (int)(p->plan.check_info[c] & 0xFFu);
             for (int j = 0; j < D; j++) rows[(size_t)c * 8 + j] = p->plan.edge_var[p->plan.check_e0[c] + j];

@@ -183,8 +183,11 @@ struct SlotState {
 // Dynamic LDS of a frame-queue workgroup: the posterior block, then the slot
 // bookkeeping and the learned table, sized for this launch (S slots, P words)
 // so that small blocks leave room for one more workgroup per CU.
+// Pair mode adds one variable past the last (index N, its pairs held at zero,
+// decision 0): the unused entries of a check's id row name it, so the
+// syndrome scan reads every row whole, with no per-entry degree test.
 __host__ __device__ inline size_t pool_post_bytes(const PoolArgs &a) {
-    return ((size_t)a.N * a.P * (a.pairs ? 2 : 4) + 7) & ~(size_t)7;
+    return ((size_t)(a.N + (a.pairs ? 1 : 0)) * a.P * (a.pairs ? 2 : 4) + 7) & ~(size_t)7;
 }
 __host__ __device__ inline int pool_slots(const PoolArgs &a) { return a.P * (a.pairs ? 2 : 4); }
 size_t pool_lds_bytes(const PoolArgs &a) {
@@ -795,7 +798,10 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
         st.fid[tid] = -1;
         st.it[tid] = 0;
     }
-    if (tid < P) s_ov[tid] = 0;
+    if (tid < P) {
+        s_ov[tid] = 0;
+        post[(uint32_t)a.N * P + tid] = 0;  // the padding variable
+    }
     __syncthreads();
 
     for (;;) {
@@ -909,7 +915,8 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                     uint32_t par = 0;
 #pragma unroll
                     for (int j = 0; j < DMAXC; j++)
-                        if (j < D) par = xor_positive(par, post[(ev[g][j] << lp) + (uint32_t)w]);
+                        // rows: entries past the degree name the padding variable (decision 0)
+                        if (ROWS || j < D) par = xor_positive(par, post[(ev[g][j] << lp) + (uint32_t)w]);
                     found |= par & 0x8080u;
                     __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
                 }
