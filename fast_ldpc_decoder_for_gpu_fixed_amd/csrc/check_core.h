@@ -245,6 +245,8 @@ __device__ __forceinline__ void store_msgs(uint32_t *__restrict__ mp, const uint
 // bases, so the loads and stores take the scalar-base + 32-bit vector-offset
 // form and no 64-bit multiply is formed per item.
 // ---------------------------------------------------------------------------
+// One batch of uniform_items: G items, all valid, at table byte offset ob and
+// message byte offset mb (the next item ostep / mstep further).
 template <int D, int G, bool EQ, bool FIRST, int VK>
 __device__ __forceinline__ uint2 uniform_batch(const char *obase, char *mbytes, char *pbase, uint32_t ob,
                                                uint32_t mb, uint32_t ostep, uint32_t mstep, int variant,
