@@ -467,6 +467,18 @@ def run_mixed(L, args, t_start):
     ms_eager = timed(L, 0, args.steps, args.warmup, lambda: step(False))
     ms = timed(L, 0, args.steps, args.warmup, lambda: step(True))
     its = d_it.download(np.empty(frames, np.int32))
+    # each code's frames decoded alone (its own frame queue, same tuning, the
+    # contiguous region of its frames): the per-code rate beside the shared step
+    alone = []
+    for k, c in enumerate(codes):
+        t = tuned[k]
+        pool = L.PoolDecoder(c, variant, max_it, words_per_block=t["words_per_block"], threads=t["threads"],
+                             uniform=t["uniform"], batch=t["batch"], pairs=t["pairs"])
+        d_o, d_i = L.DeviceBuffer(n_c[k] * c.n_vars), L.DeviceBuffer(4 * n_c[k])
+        ms_k = timed(L, pool.stream, max(5, args.steps // 4), args.warmup,
+                     lambda: pool.decode_device(d_in.ptr + base[k], n_c[k], d_o.ptr, d_i.ptr))
+        alone.append(ms_k)
+        del pool
     per_code, alg, info = [], 0, 0
     for k, c in enumerate(codes):
         ik = its[k::2].astype(np.int64)
@@ -475,7 +487,9 @@ def run_mixed(L, args, t_start):
                   "levels": c.n_levels, "frames": n_c[k], "avg_iterations": round(float(ik.mean()), 4),
                   "iterations_hist": np.bincount(ik, minlength=max_it + 1).tolist(),
                   "tuning": {kk: tuned[k][kk] for kk in ("words_per_block", "threads", "uniform", "batch", "pairs")},
-                  "tune_ms_per_32768_frames": round(tuned[k]["ms"], 4)})
+                  "tune_ms_per_32768_frames": round(tuned[k]["ms"], 4),
+                  "alone_ms": round(alone[k], 4),
+                  "alone_info_gbps": round(n_c[k] * c.n_info / (alone[k] * 1e-3) / 1e9, 3)})
         per_code.append(r)
         alg += int(ik.sum()) * (2 * c.n_edges + 2 * c.n_vars) + n_c[k] * 2 * c.n_vars
         info += n_c[k] * c.n_info
