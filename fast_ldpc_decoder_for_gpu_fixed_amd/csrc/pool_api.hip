@@ -459,7 +459,11 @@ int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, i
     for (int fmt = 1; fmt <= 2; fmt++)
     for (int P = 1; e == hipSuccess && P <= ldpc::MAX_P; P *= 2) {
         if ((size_t)code->N * (fmt == 1 ? 4 : 2) * P > ldpc::POOL_LDS_DYNAMIC) break;
-        for (int T = 64; e == hipSuccess && T <= 1024; T *= 2) {
+        // workgroup sizes in whole waves, not only powers of two: a level of
+        // n checks fills n * P / T of a round's lanes (QC 12x24 Z=81: 81
+        // checks per level, 162 items at P = 2: 63% of 256 lanes, 84% of 192)
+        for (int T : {64, 128, 192, 256, 320, 384, 512, 768, 1024}) {
+            if (e != hipSuccess) break;
             if (T % P || T < (fmt == 1 ? 4 : 2) * P) continue;
             for (int uni = fmt == 1 ? 1 : 0; uni >= 0; uni--) {
                 for (int batch : {1, 2, 4}) {

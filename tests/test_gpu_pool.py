@@ -80,7 +80,8 @@ def test_pool_parity_codes(name, variant, gpu_available):
 
 
 @pytest.mark.parametrize("P,T,uniform,batch", [(1, 64, 1, 1), (2, 128, 1, 2), (4, 256, 1, 4), (16, 512, 1, 2),
-                                              (2, 256, 0, 0), (16, 256, 0, 0)])
+                                              (2, 256, 0, 0), (16, 256, 0, 0),
+                                              (2, 192, 1, 2), (4, 320, 1, 1), (2, 192, 0, 0)])
 def test_pool_tunings_and_default_grid(P, T, uniform, batch, gpu_available):
     c = code("c36_1000")
     llr = mixed_llr(c.n_vars, 1200, seed=P)
@@ -88,6 +89,20 @@ def test_pool_tunings_and_default_grid(P, T, uniform, batch, gpu_available):
     cfg = pool.config()
     assert cfg["uniform_deg"] == (6 if uniform else 0)
     check_pool(c, L.OMS_V2, llr, 15, workgroups=0, words_per_block=P, threads=T, uniform=uniform, batch=batch)
+
+
+@pytest.mark.parametrize("name,P,T", [("qc_12x24_z27_d9", 2, 192), ("qc_4x24_z24", 1, 192), ("c36_1000", 2, 320)])
+@pytest.mark.parametrize("fmt", ["words", "pairs"])
+def test_pool_workgroups_in_whole_waves(name, P, T, fmt, gpu_available):
+    """Workgroup sizes that are whole waves but not powers of two (the
+    autotuner tries 192, 320, 384, 768): decisions and iteration counts on the
+    generic word path and in pair mode."""
+    c = code(name)
+    rate = 1.0 - c.n_checks / c.n_vars
+    llr = mixed_llr(c.n_vars, 1500, seed=T, lo=1.5, hi=4.0, rate=rate)
+    tuning = {"pairs": 1 if fmt == "pairs" else 0, "words_per_block": P, "threads": T}
+    assert L.PoolDecoder(c, L.OMS_V2, 25, **tuning).config()["threads"] == T
+    check_pool(c, L.OMS_V2, llr, 25, workgroups=3, **tuning)
 
 
 @pytest.mark.parametrize("variant", [L.NMS2, L.LOMS, L.LNMS], ids=lambda v: L.VARIANT_NAMES[v])
