@@ -97,7 +97,9 @@ ldpc_code *ldpc_code_ira(int k_info, int n_hi, int d_hi, int d_lo, uint64_t seed
  * levels.  Colouring of the check conflict graph (checks sharing a variable
  * conflict): DSatur order, each check in the least-filled colour its
  * neighbours leave free, starting from as many colours as the largest
- * variable degree; deterministic. */
+ * variable degree; then iterated greedy and tabu search cut the colour count
+ * (the tabu stage for conflict tables up to 2^22 checks x colours), and
+ * checks move from over-full classes into free smaller ones; deterministic. */
 ldpc_code *ldpc_code_reorder_by_colour(const ldpc_code *code);
 void ldpc_code_destroy(ldpc_code *code);
 
@@ -298,8 +300,10 @@ int ldpc_pool_simulate(ldpc_pool *pool, double ebn0_db, int quant, uint64_t seed
 /* Autotuner (no counterpart in the reference): times the frame-queue
  * decoder on sample_frames channel frames at ebn0_db (quantizer `quant`, as
  * ldpc_channel_generate; iteration counts depend on it) for every words-per-
- * workgroup P (posterior block N*P*4 bytes of LDS), workgroup size and, on the
- * single-degree path, items in flight per thread (1, 2, 4); writes the fastest
+ * workgroup P (posterior block N*P*4 bytes of LDS), workgroup size (whole
+ * waves: 64, 128, 192, 256, 320, 384, 512, 768, 1024, as the kernel allows)
+ * and, on the single-degree path, items in flight per thread (1, 2, 4); both
+ * the 4-frame word and the 2-frame pair formats; writes the fastest
  * into *best (words_per_block, threads, uniform, batch set; other fields -1).
  * Learned variants are timed with a constant table. */
 int ldpc_pool_autotune(const ldpc_code *code, int variant, int max_iterations, int stop_rule, double ebn0_db,
