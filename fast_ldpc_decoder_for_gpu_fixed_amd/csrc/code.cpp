@@ -90,6 +90,11 @@ int iterated_greedy(const std::vector<std::vector<int>> &nbr, std::vector<int> &
 bool tabucol(const std::vector<std::vector<int>> &nbr, std::vector<int> &colour, int k, long max_moves,
              Rng &rng) {
     const int C = (int)colour.size();
+    if (k < 2) {  // one colour: no move exists; proper only with no edge at all
+        for (int c = 0; c < C; c++)
+            if (!nbr[c].empty()) return false;
+        return true;
+    }
     std::vector<int> gamma((size_t)C * k, 0);  // gamma[c*k + x]: neighbours of c with colour x
     for (int c = 0; c < C; c++)
         for (int b : nbr[c]) gamma[(size_t)c * k + colour[b]]++;
