@@ -92,8 +92,9 @@ struct PoolArgs {
     int uniform_deg, batch;       // degree of that path, items in flight per thread (1, 2, 4)
     int stop_rule;                // 0: syndrome of the decisions, 1: v2c parity (reference rule), 2: none
     int pairs;                    // 1: posterior and messages hold 2 frames per 16 bits (P = pairs)
-    const uint32_t *rows;         // pair mode, degrees <= 8: [C][8] variable ids per check (unused
-                                  // entries 0), and messages as [grid][C][P][8] u16 runs; else null
+    const uint32_t *rows;         // pair mode, degrees <= 8: [C][8] LDS byte offsets per check
+                                  // (variable * 2P; unused entries: the padding variable N), and
+                                  // messages as [grid][C][P][8] u16 runs; else null
     const int *level_deg;         // [n_levels] degree shared by every check of the level, or 0
 };
 // largest posterior block of a frame-queue workgroup; the slot bookkeeping and
@@ -103,6 +104,7 @@ size_t pool_lds_bytes(const PoolArgs &a);
 hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipStream_t s);
 // sets the LDS limit of the instantiation `a` selects (call before any launch
 // and outside graph capture) and returns how many of its workgroups a CU holds
+// (-1: a pair-mode rows kernel with static LDS, which its addressing forbids)
 int pool_prepare(const PoolArgs &a, int dmax, int threads);
 hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t s);
 

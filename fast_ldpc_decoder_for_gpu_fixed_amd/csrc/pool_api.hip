@@ -200,7 +200,9 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     const ldpc::PoolArgs a = ldpc::pool_args(p);
     const int per_cu = ldpc::pool_prepare(a, code->max_deg, T);
     if (per_cu < 1) {
-        ldpc::set_last_error("the frame-queue kernel cannot run with this words_per_block / threads");
+        ldpc::set_last_error(per_cu < 0 ? "ldpc_pool_create: the pair-mode kernel has static LDS in front of its "
+                                          "posterior block (its rows address LDS from 0)"
+                                        : "the frame-queue kernel cannot run with this words_per_block / threads");
         pool_free(p);
         return nullptr;
     }
@@ -212,14 +214,17 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     auto al = [&](auto **q, size_t n) {
         if (e == hipSuccess && n) e = hipMalloc((void **)q, n * sizeof(**q));
     };
-    // pair mode with degrees <= 8: ids as rows of 8 per check, messages as
-    // runs of 8 u16 per check and pair (16-byte aligned, addressed by the check)
+    // pair mode with degrees <= 8: variables as rows of 8 LDS byte offsets per
+    // check (variable * 2P, the first pair of the variable's group), messages
+    // as runs of 8 u16 per check and pair (16-byte aligned, addressed by the check)
     std::vector<uint32_t> rows;
     if (p->pairs && code->max_deg <= 8) {
-        rows.assign((size_t)code->C * 8, (uint32_t)code->N);  // unused entries: the padding variable
+        const uint32_t vstride = 2u * (uint32_t)P;
+        rows.assign((size_t)code->C * 8, (uint32_t)code->N * vstride);  // unused entries: the padding variable
         for (int c = 0; c < code->C; c++) {
             const int D = (int)(p->plan.check_info[c] & 0xFFu);
-            for (int j = 0; j < D; j++) rows[(size_t)c * 8 + j] = p->plan.edge_var[p->plan.check_e0[c] + j];
+            for (int j = 0; j < D; j++)
+                rows[(size_t)c * 8 + j] = p->plan.edge_var[p->plan.check_e0[c] + j] * vstride;
         }
     }
     const size_t msg_u16 = rows.empty() ? (size_t)code->E : (size_t)code->C * 8;  // per frame group

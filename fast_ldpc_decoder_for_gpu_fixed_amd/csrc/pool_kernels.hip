@@ -551,6 +551,13 @@ __device__ __forceinline__ u16x2 pair_to_half(uint32_t v16) { return u2(__builti
 __device__ __forceinline__ uint16_t half_to_pair(u16x2 h) {
     return (uint16_t)__builtin_amdgcn_perm(0u, w32(h), 0x0C0C0301u);
 }
+// The pair at LDS byte address `addr`.  Rows-layout entries are absolute LDS
+// addresses: the posterior block is the start of the kernel's dynamic LDS,
+// which begins at address 0 because the kernel has no static LDS (the host
+// checks that before it launches one, pool_prepare), so an access needs no
+// base added and, when P = 1, no arithmetic at all.
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+__device__ __forceinline__ lds_u16 &pair_at(uint32_t addr) { return *(lds_u16 *)(uintptr_t)addr; }
 
 // One pair-mode check whose variable ids and messages are already in
 // registers (ev[j], mr[j] for j < D); mp is where its messages go back.
@@ -625,10 +632,12 @@ __device__ __forceinline__ void pair_gather(const PoolArgs &a, const uint16_t *m
     }
 }
 
-// Pair mode with degrees <= 8 (rows): a check's variable ids are one row of 8
-// and its messages one 16-byte run of 8 u16 per pair, [workgroup][check]
-// [pair][8], both addressed by the check index alone: two 16-byte loads for
-// the ids, one load and one store for the messages (entries past D unused).
+// Pair mode with degrees <= 8 (rows): a check's variables are one row of 8
+// LDS byte offsets (variable * 2P: the pair of a thread with w = 0; a thread
+// adds 2w) and its messages one 16-byte run of 8 u16 per pair, [workgroup]
+// [check][pair][8], both addressed by the check index alone: two 16-byte
+// loads for the offsets, one load and one store for the messages (entries
+// past D unused).
 template <int NG>
 __device__ __forceinline__ void rows_gather(const PoolArgs &a, const uint16_t *mbase, int c0, int cs, int le, int P,
                                             int w, uint2 (&ci)[NG], uint32_t (&ev)[NG][8], uint32_t (&mr)[NG][8]) {
@@ -668,15 +677,15 @@ __device__ __forceinline__ void rows_ids(const PoolArgs &a, int c0, int cs, int 
 // one rows-layout check: as pair_check, the 8 messages stored as one run
 // (two halves' 16-bit pairs per dword: bytes 1 and 3 of each half)
 template <int D, bool EQ>
-__device__ __forceinline__ uint32_t rows_check(uint16_t *post, const uint32_t (&ev)[8], const uint32_t (&mr)[8],
-                                               uint4 *__restrict__ mp, int lp, int w, u16x2 keep, int variant,
+__device__ __forceinline__ uint32_t rows_check(const uint32_t (&ev)[8], const uint32_t (&mr)[8],
+                                               uint4 *__restrict__ mp, uint32_t w2, u16x2 keep, int variant,
                                                u16x2 lpar) {
     uint32_t vi[D];
     u16x2 x[D], m[D], pn[D], mn[D];
 #pragma unroll
-    for (int j = 0; j < D; j++) vi[j] = (ev[j] << lp) + (uint32_t)w;
+    for (int j = 0; j < D; j++) vi[j] = ev[j] + w2;
 #pragma unroll
-    for (int j = 0; j < D; j++) x[j] = pair_to_half(post[vi[j]]);
+    for (int j = 0; j < D; j++) x[j] = pair_to_half(pair_at(vi[j]));
 #pragma unroll
     for (int j = 0; j < D; j++) m[j] = pair_to_half(mr[j]) & keep;
     const uint32_t par = check_half<D, EQ>(x, m, false, variant, false, lpar, pn, mn);
@@ -690,7 +699,7 @@ __device__ __forceinline__ uint32_t rows_check(uint16_t *post, const uint32_t (&
     o.w = __builtin_amdgcn_perm(h[7], h[6], 0x07050301u);
     *mp = o;
 #pragma unroll
-    for (int j = 0; j < D; j++) post[vi[j]] = half_to_pair(pn[j]);
+    for (int j = 0; j < D; j++) pair_at(vi[j]) = half_to_pair(pn[j]);
     return par;
 }
 // One batch of NG rows-layout checks c0, c0 + cs, ... of degree D, all valid,
@@ -700,9 +709,9 @@ __device__ __forceinline__ uint32_t rows_check(uint16_t *post, const uint32_t (&
 // high half of a loaded word into a packed-16 half with the lanes of frames
 // that start now zeroed (the keep mask folded into the unpack).
 template <int D, int NG, bool EQ>
-__device__ __forceinline__ uint32_t rows_batch(const PoolArgs &a, uint16_t *post, uint16_t *mbase, int c0, int cs,
-                                               int P, int lp, int w, uint32_t sel_lo, uint32_t sel_hi, int variant,
-                                               u16x2 lpar) {
+__device__ __forceinline__ uint32_t rows_batch(const PoolArgs &a, uint16_t *mbase, int c0, int cs,
+                                               int P, int w, uint32_t w2, uint32_t sel_lo, uint32_t sel_hi,
+                                               int variant, u16x2 lpar) {
     uint32_t ev[NG][8], mw[NG][4];
 #pragma unroll
     for (int g = 0; g < NG; g++) {
@@ -720,8 +729,8 @@ __device__ __forceinline__ uint32_t rows_batch(const PoolArgs &a, uint16_t *post
     for (int g = 0; g < NG; g++)
 #pragma unroll
         for (int j = 0; j < D; j++) {
-            vi[g][j] = (ev[g][j] << lp) + (uint32_t)w;
-            x[g][j] = pair_to_half(post[vi[g][j]]);
+            vi[g][j] = ev[g][j] + w2;
+            x[g][j] = pair_to_half(pair_at(vi[g][j]));
         }
     uint32_t ov = 0;
 #pragma unroll
@@ -740,7 +749,7 @@ __device__ __forceinline__ uint32_t rows_batch(const PoolArgs &a, uint16_t *post
         o.w = __builtin_amdgcn_perm(h[7], h[6], 0x07050301u);
         *reinterpret_cast<uint4 *>(mbase + ((size_t)(c0 + g * cs) * P + w) * 8) = o;
 #pragma unroll
-        for (int j = 0; j < D; j++) post[vi[g][j]] = half_to_pair(pn[j]);
+        for (int j = 0; j < D; j++) pair_at(vi[g][j]) = half_to_pair(pn[j]);
     }
     return ov;
 }
@@ -749,25 +758,25 @@ __device__ __forceinline__ uint32_t rows_batch(const PoolArgs &a, uint16_t *post
 // takes its checks c0, c0 + cs, ... in batches of NG while NG of them remain,
 // then one at a time, so no lane computes a check it does not own.
 template <int D, int NG, bool EQ>
-__device__ __forceinline__ uint32_t rows_level(const PoolArgs &a, uint16_t *post, uint16_t *mbase, int lb, int le,
-                                               int cs, int P, int lp, int w, uint32_t sel_lo, uint32_t sel_hi,
-                                               int variant, u16x2 lpar) {
+__device__ __forceinline__ uint32_t rows_level(const PoolArgs &a, uint16_t *mbase, int lb, int le,
+                                               int cs, int P, int lp, int w, uint32_t w2, uint32_t sel_lo,
+                                               uint32_t sel_hi, int variant, u16x2 lpar) {
     uint32_t ov = 0;
     int c0 = lb + ((int)threadIdx.x >> lp);
     for (; c0 + (NG - 1) * cs < le; c0 += NG * cs)
-        ov |= rows_batch<D, NG, EQ>(a, post, mbase, c0, cs, P, lp, w, sel_lo, sel_hi, variant, lpar);
+        ov |= rows_batch<D, NG, EQ>(a, mbase, c0, cs, P, w, w2, sel_lo, sel_hi, variant, lpar);
     for (; c0 < le; c0 += cs)
-        ov |= rows_batch<D, 1, EQ>(a, post, mbase, c0, cs, P, lp, w, sel_lo, sel_hi, variant, lpar);
+        ov |= rows_batch<D, 1, EQ>(a, mbase, c0, cs, P, w, w2, sel_lo, sel_hi, variant, lpar);
     return ov;
 }
 
 template <bool EQ>
-__device__ __forceinline__ uint32_t rows_check_any(int D, uint16_t *post, const uint32_t (&ev)[8],
-                                                   const uint32_t (&mr)[8], uint4 *__restrict__ mp, int lp, int w,
+__device__ __forceinline__ uint32_t rows_check_any(int D, const uint32_t (&ev)[8],
+                                                   const uint32_t (&mr)[8], uint4 *__restrict__ mp, uint32_t w2,
                                                    u16x2 keep, int variant, u16x2 lpar) {
     switch (D) {
 #define LDPC_RCASE(d) \
-    case d: return rows_check<d, EQ>(post, ev, mr, mp, lp, w, keep, variant, lpar);
+    case d: return rows_check<d, EQ>(ev, mr, mp, w2, keep, variant, lpar);
         LDPC_RCASE(1) LDPC_RCASE(2) LDPC_RCASE(3) LDPC_RCASE(4) LDPC_RCASE(5) LDPC_RCASE(6) LDPC_RCASE(7)
         LDPC_RCASE(8)
 #undef LDPC_RCASE
@@ -775,7 +784,9 @@ __device__ __forceinline__ uint32_t rows_check_any(int D, uint16_t *post, const 
     }
 }
 
-template <bool EQ, int DMAXC>
+// P1: one pair per workgroup (a.P == 1), so a thread's pair offset w is 0 at
+// compile time and a rows-layout entry is the LDS address itself.
+template <bool EQ, int DMAXC, bool P1>
 __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const PoolArgs a) {
     extern __shared__ uint16_t lds_post16[];
     const SlotState st = carve_slots(a, reinterpret_cast<char *>(lds_post16));
@@ -783,9 +794,10 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
     uint32_t *const s_ov = st.ove;
     int8_t *const s_tab = st.tab;
 
-    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P;
+    const int tid = threadIdx.x, T = blockDim.x, P = P1 ? 1 : a.P, lp = P1 ? 0 : a.log2P;
     const int S = 2 * P;
     const int w = tid & (P - 1);
+    const uint32_t w2 = 2u * (uint32_t)w;  // byte offset of the thread's pair within a variable
     const int cstride = T >> lp;
     uint16_t *post = lds_post16;
     constexpr bool ROWS = DMAXC == 8;  // degrees <= 8: the rows layout (the host provides a.rows)
@@ -854,7 +866,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                     switch (ud) {
 #define LDPC_RLEVEL(d)                                                                                          \
     case d:                                                                                                     \
-        ov |= rows_level<d, NG, EQ>(a, post, mbase, lb, le, cstride, P, lp, w, sel_lo, sel_hi, a.variant, lpar); \
+        ov |= rows_level<d, NG, EQ>(a, mbase, lb, le, cstride, P, lp, w, w2, sel_lo, sel_hi, a.variant, lpar); \
         break;
                         LDPC_RLEVEL(4) LDPC_RLEVEL(5) LDPC_RLEVEL(6) LDPC_RLEVEL(7) LDPC_RLEVEL(8)
 #undef LDPC_RLEVEL
@@ -881,9 +893,9 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                                   (learned_half(a.variant, s_tab[rowoff[1] + col]) << 16));
                     }
                     if constexpr (ROWS)
-                        ov |= rows_check_any<EQ>(D, post, ev[g], mr[g],
+                        ov |= rows_check_any<EQ>(D, ev[g], mr[g],
                                                  reinterpret_cast<uint4 *>(mbase + ((size_t)(c0 + g * cstride) * P + w) * 8),
-                                                 lp, w, keep, a.variant, lpar);
+                                                 w2, keep, a.variant, lpar);
                     else
                         ov |= pair_check_any<EQ, DMAXC>(D, post, ev[g], mr[g],
                                                         mbase + (size_t)ci[g].y * P + (size_t)w * D, lp, w, keep,
@@ -911,13 +923,24 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                 uint32_t found = 0;
 #pragma unroll
                 for (int g = 0; g < SG; g++) {
-                    const int D = (int)(ci[g].x & 0xFFu);
                     uint32_t par = 0;
+                    if constexpr (ROWS) {
+                        // two pairs per test (bytes 0-1: entry j, 2-3: entry j + 1), the
+                        // halves folded once per check; entries past the degree name
+                        // the padding variable (decision 0)
 #pragma unroll
-                    for (int j = 0; j < DMAXC; j++)
-                        // rows: entries past the degree name the padding variable (decision 0)
-                        if (ROWS || j < D) par = xor_positive(par, post[(ev[g][j] << lp) + (uint32_t)w]);
-                    found |= par & 0x8080u;
+                        for (int j = 0; j < 8; j += 2)
+                            par = xor_positive(par, (uint32_t)pair_at(ev[g][j] + w2) |
+                                                        ((uint32_t)pair_at(ev[g][j + 1] + w2) << 16));
+                        // (par ^ par >> 16) & 0x8080 (truth table 0x28: (a ^ b) & c)
+                        found |= __builtin_amdgcn_bitop3_b32(par, par >> 16, 0x8080u, 0x28);
+                    } else {
+                        const int D = (int)(ci[g].x & 0xFFu);
+#pragma unroll
+                        for (int j = 0; j < DMAXC; j++)
+                            if (j < D) par = xor_positive(par, post[(ev[g][j] << lp) + (uint32_t)w]);
+                        found |= par & 0x8080u;
+                    }
                     __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
                 }
                 if (found & ~ov) atomicOr(&s_ov[w], found);
@@ -931,10 +954,10 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
 }
 
 template <bool EQ>
-static const void *pairs_kernel(int dmax) {
-    if (dmax <= 8) return (const void *)k_pool_pairs<EQ, 8>;
-    if (dmax <= 16) return (const void *)k_pool_pairs<EQ, 16>;
-    return (const void *)k_pool_pairs<EQ, 32>;
+static const void *pairs_kernel(int dmax, int P) {
+    if (dmax <= 8) return P == 1 ? (const void *)k_pool_pairs<EQ, 8, true> : (const void *)k_pool_pairs<EQ, 8, false>;
+    if (dmax <= 16) return (const void *)k_pool_pairs<EQ, 16, false>;
+    return (const void *)k_pool_pairs<EQ, 32, false>;
 }
 
 template <bool EQ, int VK>
@@ -953,7 +976,7 @@ static const void *uniform_kernel(int ud, int G) {
 // degree classes.
 static const void *pool_kernel(const PoolArgs &a, int dmax) {
     const bool eq = a.variant == VAR_2NMS;
-    if (a.pairs) return eq ? pairs_kernel<true>(dmax) : pairs_kernel<false>(dmax);
+    if (a.pairs) return eq ? pairs_kernel<true>(dmax, a.P) : pairs_kernel<false>(dmax, a.P);
     if (a.uniform_deg) {
         if (!uniform_degree_supported(a.uniform_deg) || a.uniform_deg != dmax) return nullptr;
         if (eq) return uniform_kernel<true, -1>(a.uniform_deg, a.batch);
@@ -982,6 +1005,12 @@ int pool_prepare(const PoolArgs &a, int dmax, int threads) {
     int n = 0;
     const void *f = pool_kernel(a, dmax);
     if (!f) return 0;
+    if (a.pairs && dmax <= 8) {
+        // the rows layout addresses LDS absolutely from 0 (pair_at): the
+        // kernel must have no static LDS in front of its dynamic block
+        hipFuncAttributes fa;
+        if (hipFuncGetAttributes(&fa, f) != hipSuccess || fa.sharedSizeBytes != 0) return -1;
+    }
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)MAX_LDS_BYTES) != hipSuccess)
         return 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, pool_lds_bytes(a)) != hipSuccess) return 0;
