@@ -73,6 +73,8 @@ hipError_t launch_deinterleave_raw(const uint32_t *in, int8_t *out, int N, int W
 struct PoolArgs {
     const uint2 *checks;
     const uint32_t *edge_var;
+    const uint32_t *edge_lds;  // [E] + GATHER_PAD: LDS byte address of each level-order edge's variable
+                               // (variable * 4P for words, * 2P for pairs)
     const int *level_beg;
     int n_levels, N, C, E;
     int P, log2P;

@@ -13,6 +13,7 @@ struct ldpc_pool {
     hipStream_t stream = nullptr;
     uint2 *d_checks = nullptr;
     uint32_t *d_edge_var = nullptr;
+    uint32_t *d_edge_lds = nullptr;  // LDS byte address of each edge's variable (the kernels' gathers)
     int *d_level_beg = nullptr;
     uint32_t *d_msgs = nullptr;
     unsigned long long *d_next = nullptr;
@@ -22,7 +23,7 @@ struct ldpc_pool {
     int uniform_deg = 0, batch = 1;
     int stop_rule = 0;
     int pairs = 0;  // posterior and messages as 2-frame pairs (P counts pairs)
-    uint32_t *d_rows = nullptr;  // pair mode with degrees <= 8: ids as rows of 8 per check
+    uint32_t *d_rows = nullptr;  // pair mode with degrees <= 8: LDS addresses as rows of 8 per check
     int *d_level_deg = nullptr;  // degree shared by every check of a level, or 0
     int n_cu = 1;   // compute units of the device
     // captured decode step (ldpc_pool_decode_graph) and what it was captured with

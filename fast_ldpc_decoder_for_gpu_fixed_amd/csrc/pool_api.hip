@@ -39,6 +39,7 @@ void pool_free(ldpc_pool *p) {
     if (p->capture) (void)hipStreamDestroy(p->capture);
     (void)hipFree(p->d_checks);
     (void)hipFree(p->d_edge_var);
+    (void)hipFree(p->d_edge_lds);
     (void)hipFree(p->d_level_beg);
     (void)hipFree(p->d_msgs);
     (void)hipFree(p->d_next);
@@ -59,6 +60,7 @@ PoolArgs pool_args(const ldpc_pool *p) {
     std::memset(&a, 0, sizeof a);
     a.checks = p->d_checks;
     a.edge_var = p->d_edge_var;
+    a.edge_lds = p->d_edge_lds;
     a.level_beg = p->d_level_beg;
     a.n_levels = p->plan.n_levels;
     a.N = p->code.N;
@@ -217,9 +219,12 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     // pair mode with degrees <= 8: variables as rows of 8 LDS byte offsets per
     // check (variable * 2P, the first pair of the variable's group), messages
     // as runs of 8 u16 per check and pair (16-byte aligned, addressed by the check)
+    // LDS byte address of each edge's variable (its first word or pair)
+    const uint32_t vstride = (p->pairs ? 2u : 4u) * (uint32_t)P;
+    std::vector<uint32_t> edge_lds((size_t)code->E + ldpc::GATHER_PAD, 0u);
+    for (size_t i = 0; i < (size_t)code->E; i++) edge_lds[i] = p->plan.edge_var[i] * vstride;
     std::vector<uint32_t> rows;
     if (p->pairs && code->max_deg <= 8) {
-        const uint32_t vstride = 2u * (uint32_t)P;
         rows.assign((size_t)code->C * 8, (uint32_t)code->N * vstride);  // unused entries: the padding variable
         for (int c = 0; c < code->C; c++) {
             const int D = (int)(p->plan.check_info[c] & 0xFFu);
@@ -230,6 +235,7 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     const size_t msg_u16 = rows.empty() ? (size_t)code->E : (size_t)code->C * 8;  // per frame group
     al(&p->d_checks, ck.size());
     al(&p->d_edge_var, (size_t)code->E + ldpc::GATHER_PAD);
+    al(&p->d_edge_lds, edge_lds.size());
     al(&p->d_level_beg, p->plan.level_beg.size());
     al(&p->d_level_deg, p->plan.level_deg.size());
     al(&p->d_msgs, ((size_t)p->max_grid * msg_u16 * P * wb / code->N + 3) / 4 + ldpc::GATHER_PAD);  // u32 units
@@ -240,6 +246,7 @@ ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iteratio
     if (e == hipSuccess) e = hipMemcpy(p->d_checks, ck.data(), ck.size() * sizeof(uint2), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_edge_var, p->plan.edge_var.data(), (size_t)code->E * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(p->d_edge_var + code->E, 0, (size_t)ldpc::GATHER_PAD * 4);
+    if (e == hipSuccess) e = hipMemcpy(p->d_edge_lds, edge_lds.data(), edge_lds.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(p->d_level_beg, p->plan.level_beg.data(), p->plan.level_beg.size() * sizeof(int),
                       hipMemcpyHostToDevice);

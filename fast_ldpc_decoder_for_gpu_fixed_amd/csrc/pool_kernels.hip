@@ -20,21 +20,33 @@
 
 namespace ldpc {
 
+// LDS by absolute byte address.  The per-edge tables of the generic paths
+// (PoolArgs::edge_lds) and the pair-mode rows hold LDS addresses of the
+// posterior block, which starts the kernel's dynamic LDS at address 0: the
+// frame-queue kernels have no static LDS (pool_prepare checks it before a
+// launch), so an access adds no base and, when P = 1, no offset at all.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+__device__ __forceinline__ lds_u32 &word_at(uint32_t addr) { return *(lds_u32 *)(uintptr_t)addr; }
+__device__ __forceinline__ lds_u16 &pair_at(uint32_t addr) { return *(lds_u16 *)(uintptr_t)addr; }
+
+// one check of degree D on a word block: ev are the LDS addresses of its
+// variables' first word, w4 the thread's word offset (4w)
 template <int D, bool EQ>
-__device__ __forceinline__ uint2 pool_check(uint32_t *post, const uint32_t *__restrict__ ev, uint32_t *__restrict__ mp,
-                                            int lp, int w, uint32_t keep, int variant, u16x2 lpe, u16x2 lpo) {
+__device__ __forceinline__ uint2 pool_check(const uint32_t *__restrict__ ev, uint32_t *__restrict__ mp, uint32_t w4,
+                                            uint32_t keep, int variant, u16x2 lpe, u16x2 lpo) {
     uint32_t vi[D], pw[D], mw[D], pn[D], mn[D];
 #pragma unroll
-    for (int j = 0; j < D; j++) vi[j] = (ev[j] << lp) + (uint32_t)w;
+    for (int j = 0; j < D; j++) vi[j] = ev[j] + w4;
 #pragma unroll
-    for (int j = 0; j < D; j++) pw[j] = post[vi[j]];
+    for (int j = 0; j < D; j++) pw[j] = word_at(vi[j]);
 #pragma unroll
     for (int j = 0; j < D; j++) mw[j] = mp[j];
     const uint2 pr = check_core<D, EQ>(pw, mw, false, variant, false, lpe, lpo, pn, mn, msg_unpack(keep));
 #pragma unroll
     for (int j = 0; j < D; j++) {
         mp[j] = mn[j];
-        post[vi[j]] = pn[j];
+        word_at(vi[j]) = pn[j];
     }
     return pr;
 }
@@ -120,14 +132,13 @@ __device__ __forceinline__ uint2 syndrome_items(const PoolArgs &a, const uint32_
 }
 
 template <bool EQ, int DMAXC>
-__device__ __forceinline__ uint2 pool_check_any(int D, uint32_t *post, const uint32_t *__restrict__ ev,
-                                                uint32_t *__restrict__ mp, int lp, int w, uint32_t keep, int variant,
-                                                u16x2 lpe, u16x2 lpo) {
+__device__ __forceinline__ uint2 pool_check_any(int D, const uint32_t *__restrict__ ev, uint32_t *__restrict__ mp,
+                                                uint32_t w4, uint32_t keep, int variant, u16x2 lpe, u16x2 lpo) {
     switch (D) {
 #define LDPC_PCASE(d)                                                                        \
     case d:                                                                                  \
         if constexpr (d <= DMAXC)                                                            \
-            return pool_check<d, EQ>(post, ev, mp, lp, w, keep, variant, lpe, lpo);          \
+            return pool_check<d, EQ>(ev, mp, w4, keep, variant, lpe, lpo);                   \
         else                                                                                 \
             return make_uint2(0u, 0u);
         LDPC_PCASE(1) LDPC_PCASE(2) LDPC_PCASE(3) LDPC_PCASE(4) LDPC_PCASE(5) LDPC_PCASE(6)
@@ -141,11 +152,11 @@ __device__ __forceinline__ uint2 pool_check_any(int D, uint32_t *post, const uin
     }
 }
 
-// Entries (degree, first edge) and variable ids of NG checks c0, c0 + cs, ...
-// below `le` (a missing check gets D = 0), all loads issued before any is
-// used.  DMAXC ids are read whatever D is (those past D belong to the next
-// check and are never used); the id array is padded so that the last
-// check's reads stay inside it.
+// Entries (degree, first edge) and variable LDS addresses of NG checks c0,
+// c0 + cs, ... below `le` (a missing check gets D = 0), all loads issued
+// before any is used.  DMAXC addresses are read whatever D is (those past D
+// belong to the next check and are never used); the table is padded so that
+// the last check's reads stay inside it.
 template <int NG, int DMAXC>
 __device__ __forceinline__ void gather_ids(const PoolArgs &a, int c0, int cs, int le, uint2 (&ci)[NG],
                                            uint32_t (&ev)[NG][DMAXC]) {
@@ -156,7 +167,7 @@ __device__ __forceinline__ void gather_ids(const PoolArgs &a, int c0, int cs, in
     }
 #pragma unroll
     for (int g = 0; g < NG; g++) {
-        const uint32_t *e = a.edge_var + ci[g].y;
+        const uint32_t *e = a.edge_lds + ci[g].y;
 #pragma unroll
         for (int j = 0; j < DMAXC; j++) ev[g][j] = e[j];
     }
@@ -380,10 +391,11 @@ __device__ __forceinline__ void pool_io(const PoolArgs &a, const SlotState &st, 
     }
 }
 
-template <bool EQ, int DMAXC, int UD, int G, int VK>
+template <bool EQ, int DMAXC, int UD, int G, int VK, bool P1 = false>
 // Register budgets of the generic path: degrees <= 10 fit 128 VGPRs (4 waves
 // per SIMD, 125 used), degrees <= 16 are held to 168 (3 waves instead of the
-// 2 that 176 unconstrained registers give); neither spills.
+// 2 that 176 unconstrained registers give); neither spills.  P1: one word per
+// workgroup (a.P == 1), the thread's word offset 0 at compile time.
 __global__ __launch_bounds__(UD > 0 ? 512 : (DMAXC <= 8 ? 1024 : 256))
 __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 10 ? 4 : UD == 0 && DMAXC == 16 ? 3 : 1))) void k_pool(
     const PoolArgs a) {
@@ -392,9 +404,10 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 10 ? 4 : UD == 0 && DMAXC
     uint32_t *const s_keep = st.keep, *const s_ove = st.ove, *const s_ovo = st.ovo;
     int8_t *const s_tab = st.tab;
 
-    const int tid = threadIdx.x, T = blockDim.x, P = a.P, lp = a.log2P;
+    const int tid = threadIdx.x, T = blockDim.x, P = P1 ? 1 : a.P, lp = P1 ? 0 : a.log2P;
     const int S = 4 * P;
     const int w = tid & (P - 1);
+    const uint32_t w4 = 4u * (uint32_t)w;  // byte offset of the thread's word within a variable
     const int cstride = T >> lp;
     uint32_t *post = lds_post;
     uint32_t *mbase = a.msgs + (size_t)blockIdx.x * a.E * P;
@@ -473,8 +486,8 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 10 ? 4 : UD == 0 && DMAXC
                         lpe = u2(l[0] | (l[2] << 16));
                         lpo = u2(l[1] | (l[3] << 16));
                     }
-                    const uint2 pr = pool_check_any<EQ, DMAXC>(D, post, a.edge_var + ci.y,
-                                                               mbase + (size_t)ci.y * P + (size_t)w * D, lp, w, keep,
+                    const uint2 pr = pool_check_any<EQ, DMAXC>(D, a.edge_lds + ci.y,
+                                                               mbase + (size_t)ci.y * P + (size_t)w * D, w4, keep,
                                                                a.variant, lpe, lpo);
                     ove |= pr.x;
                     ovo |= pr.y;
@@ -509,7 +522,7 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 10 ? 4 : UD == 0 && DMAXC
                     uint32_t par = 0;
 #pragma unroll
                     for (int j = 0; j < DMAXC; j++)
-                        if (j < D) par = xor_positive(par, post[(ev[g][j] << lp) + (uint32_t)w]);
+                        if (j < D) par = xor_positive(par, word_at(ev[g][j] + w4));
                     found |= par & 0x80808080u;
                     __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
                 }
@@ -551,46 +564,39 @@ __device__ __forceinline__ u16x2 pair_to_half(uint32_t v16) { return u2(__builti
 __device__ __forceinline__ uint16_t half_to_pair(u16x2 h) {
     return (uint16_t)__builtin_amdgcn_perm(0u, w32(h), 0x0C0C0301u);
 }
-// The pair at LDS byte address `addr`.  Rows-layout entries are absolute LDS
-// addresses: the posterior block is the start of the kernel's dynamic LDS,
-// which begins at address 0 because the kernel has no static LDS (the host
-// checks that before it launches one, pool_prepare), so an access needs no
-// base added and, when P = 1, no arithmetic at all.
-typedef __attribute__((address_space(3))) uint16_t lds_u16;
-__device__ __forceinline__ lds_u16 &pair_at(uint32_t addr) { return *(lds_u16 *)(uintptr_t)addr; }
 
-// One pair-mode check whose variable ids and messages are already in
-// registers (ev[j], mr[j] for j < D); mp is where its messages go back.
+// One pair-mode check whose variable LDS addresses and messages are already
+// in registers (ev[j], mr[j] for j < D); mp is where its messages go back.
 template <int D, bool EQ, int DM>
-__device__ __forceinline__ uint32_t pair_check(uint16_t *post, const uint32_t (&ev)[DM], const uint32_t (&mr)[DM],
-                                               uint16_t *__restrict__ mp, int lp, int w, u16x2 keep, int variant,
+__device__ __forceinline__ uint32_t pair_check(const uint32_t (&ev)[DM], const uint32_t (&mr)[DM],
+                                               uint16_t *__restrict__ mp, uint32_t w2, u16x2 keep, int variant,
                                                u16x2 lpar) {
     uint32_t vi[D];
     u16x2 x[D], m[D], pn[D], mn[D];
 #pragma unroll
-    for (int j = 0; j < D; j++) vi[j] = (ev[j] << lp) + (uint32_t)w;
+    for (int j = 0; j < D; j++) vi[j] = ev[j] + w2;
 #pragma unroll
-    for (int j = 0; j < D; j++) x[j] = pair_to_half(post[vi[j]]);
+    for (int j = 0; j < D; j++) x[j] = pair_to_half(pair_at(vi[j]));
 #pragma unroll
     for (int j = 0; j < D; j++) m[j] = pair_to_half(mr[j]) & keep;
     const uint32_t par = check_half<D, EQ>(x, m, false, variant, false, lpar, pn, mn);
 #pragma unroll
     for (int j = 0; j < D; j++) {
         mp[j] = half_to_pair(mn[j]);
-        post[vi[j]] = half_to_pair(pn[j]);
+        pair_at(vi[j]) = half_to_pair(pn[j]);
     }
     return par;
 }
 
 template <bool EQ, int DMAXC>
-__device__ __forceinline__ uint32_t pair_check_any(int D, uint16_t *post, const uint32_t (&ev)[DMAXC],
-                                                   const uint32_t (&mr)[DMAXC], uint16_t *__restrict__ mp, int lp,
-                                                   int w, u16x2 keep, int variant, u16x2 lpar) {
+__device__ __forceinline__ uint32_t pair_check_any(int D, const uint32_t (&ev)[DMAXC], const uint32_t (&mr)[DMAXC],
+                                                   uint16_t *__restrict__ mp, uint32_t w2, u16x2 keep, int variant,
+                                                   u16x2 lpar) {
     switch (D) {
 #define LDPC_QCASE(d)                                                                        \
     case d:                                                                                  \
         if constexpr (d <= DMAXC)                                                            \
-            return pair_check<d, EQ, DMAXC>(post, ev, mr, mp, lp, w, keep, variant, lpar);   \
+            return pair_check<d, EQ, DMAXC>(ev, mr, mp, w2, keep, variant, lpar);            \
         else                                                                                 \
             return 0u;
         LDPC_QCASE(1) LDPC_QCASE(2) LDPC_QCASE(3) LDPC_QCASE(4) LDPC_QCASE(5) LDPC_QCASE(6)
@@ -622,7 +628,7 @@ __device__ __forceinline__ void pair_gather(const PoolArgs &a, const uint16_t *m
 #pragma unroll
     for (int g = 0; g < NG; g++) {
         const int D = (int)(ci[g].x & 0xFFu);
-        const uint32_t *e = a.edge_var + ci[g].y;
+        const uint32_t *e = a.edge_lds + ci[g].y;
         const uint16_t *m = mbase + (size_t)ci[g].y * P + (size_t)w * D;
 #pragma unroll
         for (int j = 0; j < DMAXC; j++) {
@@ -897,8 +903,8 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                                                  reinterpret_cast<uint4 *>(mbase + ((size_t)(c0 + g * cstride) * P + w) * 8),
                                                  w2, keep, a.variant, lpar);
                     else
-                        ov |= pair_check_any<EQ, DMAXC>(D, post, ev[g], mr[g],
-                                                        mbase + (size_t)ci[g].y * P + (size_t)w * D, lp, w, keep,
+                        ov |= pair_check_any<EQ, DMAXC>(D, ev[g], mr[g],
+                                                        mbase + (size_t)ci[g].y * P + (size_t)w * D, w2, keep,
                                                         a.variant, lpar);
                 }
             }
@@ -938,7 +944,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                         const int D = (int)(ci[g].x & 0xFFu);
 #pragma unroll
                         for (int j = 0; j < DMAXC; j++)
-                            if (j < D) par = xor_positive(par, post[(ev[g][j] << lp) + (uint32_t)w]);
+                            if (j < D) par = xor_positive(par, pair_at(ev[g][j] + w2));
                         found |= par & 0x8080u;
                     }
                     __builtin_amdgcn_sched_barrier(0);  // one check at a time (registers)
@@ -984,9 +990,13 @@ static const void *pool_kernel(const PoolArgs &a, int dmax) {
         if (a.variant == VAR_LOMS) return uniform_kernel<false, VAR_LOMS>(a.uniform_deg, a.batch);
         return uniform_kernel<false, -1>(a.uniform_deg, a.batch);
     }
-    if (dmax <= 8) return eq ? (const void *)k_pool<true, 8, 0, 1, -1> : (const void *)k_pool<false, 8, 0, 1, -1>;
-    if (dmax <= 10) return eq ? (const void *)k_pool<true, 10, 0, 1, -1> : (const void *)k_pool<false, 10, 0, 1, -1>;
-    if (dmax <= 16) return eq ? (const void *)k_pool<true, 16, 0, 1, -1> : (const void *)k_pool<false, 16, 0, 1, -1>;
+    const bool p1 = a.P == 1;
+#define LDPC_GK(DM)                                                                                     \
+    if (dmax <= DM)                                                                                     \
+        return eq ? (p1 ? (const void *)k_pool<true, DM, 0, 1, -1, true> : (const void *)k_pool<true, DM, 0, 1, -1>) \
+                  : (p1 ? (const void *)k_pool<false, DM, 0, 1, -1, true> : (const void *)k_pool<false, DM, 0, 1, -1>);
+    LDPC_GK(8) LDPC_GK(10) LDPC_GK(16)
+#undef LDPC_GK
     return eq ? (const void *)k_pool<true, 32, 0, 1, -1> : (const void *)k_pool<false, 32, 0, 1, -1>;
 }
 
@@ -1005,9 +1015,9 @@ int pool_prepare(const PoolArgs &a, int dmax, int threads) {
     int n = 0;
     const void *f = pool_kernel(a, dmax);
     if (!f) return 0;
-    if (a.pairs && dmax <= 8) {
-        // the rows layout addresses LDS absolutely from 0 (pair_at): the
-        // kernel must have no static LDS in front of its dynamic block
+    {
+        // the edge tables and rows address LDS absolutely from 0 (word_at,
+        // pair_at): no static LDS may sit in front of the dynamic block
         hipFuncAttributes fa;
         if (hipFuncGetAttributes(&fa, f) != hipSuccess || fa.sharedSizeBytes != 0) return -1;
     }
