@@ -83,7 +83,12 @@ __device__ __forceinline__ void magnitudes(int variant, u16x2 min1, u16x2 min2, 
 template <int D, bool EQ>
 __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 (&m)[D], bool first, int variant,
                                                bool nocap, u16x2 lp, u16x2 (&pn)[D], u16x2 (&mn)[D]) {
-    constexpr bool KEEP = D <= 12;  // keep |t| and -t live across the two passes
+    // KEEP: |t| and the sign of t stay live across the two passes (D <= 12);
+    // larger checks recompute them in the second pass to bound registers.
+    // With KEEP the sign is kept as a mask and the message's sign applied by
+    // a multiply by +-1; without it, as the top bit and a conditional negate
+    // (the mask form costs one op more per edge when recomputed).
+    constexpr bool KEEP = D <= 12;
     u16x2 t[D], ng[KEEP ? D : 1], ab[KEEP ? D : 1];
     u16x2 m1 = splat(0x7F00u), m2 = m1;
     u16x2 sg = splat(0u), yp = splat(0u);
@@ -94,9 +99,11 @@ __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 
         if (!first) v = sub_sat_s(v, m[j]);
         if (EQ) v &= splat(0xFF00u);
         t[j] = v;
-        const u16x2 y = sub_sat_s(zero, v);  // -t, top bit = (t > 0)
-        const u16x2 a = umin(v, y);          // |t| as uint
-        // parity of the positives: two edges per 3-input XOR (v_bitop3 0x96)
+        u16x2 y = sub_sat_s(zero, v);  // -t, top bit = (t > 0)
+        const u16x2 a = umin(v, y);    // |t| as uint
+        if constexpr (KEEP) y = sign_mask(y);  // 0xFFFF where t > 0
+        // parity of the positives (top bit; with KEEP every bit): two edges
+        // per 3-input XOR (v_bitop3 0x96)
         if (j & 1)
             sg = u2(__builtin_amdgcn_bitop3_b32(w32(sg), w32(yp), w32(y), 0x96));
         else if (j == D - 1)
@@ -122,8 +129,8 @@ __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 
     // k1 - min(a - min1, k1 - k2) == max(sat_u(k1 + min1 - a), k2): one op
     // less per edge (k1 + min1 <= 0xFE00 cannot wrap)
     const u16x2 C1 = k1 + m1;
-    sg ^= splat(0x8000u);  // top bit now: parity of the others is even -> negate
-    {
+    if constexpr (!KEEP) {
+        sg ^= splat(0x8000u);  // top bit now: parity of the others is even -> negate
         // opaque, or the compiler re-associates (sg ^ c) ^ y into (sg ^ y) ^ c:
         // one XOR more per edge
         uint32_t sw = w32(sg);
@@ -148,8 +155,17 @@ __device__ __forceinline__ uint32_t check_half(const u16x2 (&x)[D], const u16x2 
         } else {
             r = umax(sub_sat_u(C1, a), k2);
         }
-        const u16x2 sj = sign_mask(sg ^ y);
-        mn[j] = (r ^ sj) - sj;
+        // r has a clean low byte (so has -r)
+        if constexpr (KEEP) {
+            // negative where the parity of the others' positives is even:
+            // s = ~(sg ^ y) | 1 is -1 there and +1 elsewhere (one v_bitop3,
+            // truth table 0xEB), msg = r * s (one v_pk_mul_lo_u16)
+            const u16x2 s = u2(__builtin_amdgcn_bitop3_b32(w32(sg), w32(y), 0x00010001u, 0xEB));
+            mn[j] = r * s;
+        } else {
+            const u16x2 sj = sign_mask(sg ^ y);
+            mn[j] = (r ^ sj) - sj;
+        }
         pn[j] = add_sat_s(t[j], mn[j]);
     }
     return parity;
