@@ -51,7 +51,7 @@ TABLE = ROOT / "fast_ldpc_decoder_for_gpu_fixed_amd" / "tables" / "loms_3_6_n400
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=["matched", "plain", "mixed", "ira"], default="matched")
     ap.add_argument("--variant", default="OMS", help="plain mode: MS, OMS, OMS_V2, NMS, 2NMS")
