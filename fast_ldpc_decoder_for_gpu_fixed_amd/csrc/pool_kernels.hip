@@ -791,8 +791,10 @@ __device__ __forceinline__ uint32_t rows_check_any(int D, const uint32_t (&ev)[8
 }
 
 // P1: one pair per workgroup (a.P == 1), so a thread's pair offset w is 0 at
-// compile time and a rows-layout entry is the LDS address itself.
-template <bool EQ, int DMAXC, bool P1>
+// compile time and a rows-layout entry is the LDS address itself.  VK >= 0:
+// the variant known at compile time (the magnitude map becomes straight-line
+// code; a runtime switch splits every check into basic blocks), -1: a.variant.
+template <bool EQ, int DMAXC, bool P1, int VK = -1>
 __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const PoolArgs a) {
     extern __shared__ uint16_t lds_post16[];
     const SlotState st = carve_slots(a, reinterpret_cast<char *>(lds_post16));
@@ -802,6 +804,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
 
     const int tid = threadIdx.x, T = blockDim.x, P = P1 ? 1 : a.P, lp = P1 ? 0 : a.log2P;
     const int S = 2 * P;
+    const int variant = VK >= 0 ? VK : a.variant;
     const int w = tid & (P - 1);
     const uint32_t w2 = 2u * (uint32_t)w;  // byte offset of the thread's pair within a variable
     const int cstride = T >> lp;
@@ -872,7 +875,7 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                     switch (ud) {
 #define LDPC_RLEVEL(d)                                                                                          \
     case d:                                                                                                     \
-        ov |= rows_level<d, NG, EQ>(a, mbase, lb, le, cstride, P, lp, w, w2, sel_lo, sel_hi, a.variant, lpar); \
+        ov |= rows_level<d, NG, EQ>(a, mbase, lb, le, cstride, P, lp, w, w2, sel_lo, sel_hi, variant, lpar);   \
         break;
                         LDPC_RLEVEL(4) LDPC_RLEVEL(5) LDPC_RLEVEL(6) LDPC_RLEVEL(7) LDPC_RLEVEL(8)
 #undef LDPC_RLEVEL
@@ -901,11 +904,11 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
                     if constexpr (ROWS)
                         ov |= rows_check_any<EQ>(D, ev[g], mr[g],
                                                  reinterpret_cast<uint4 *>(mbase + ((size_t)(c0 + g * cstride) * P + w) * 8),
-                                                 w2, keep, a.variant, lpar);
+                                                 w2, keep, variant, lpar);
                     else
                         ov |= pair_check_any<EQ, DMAXC>(D, ev[g], mr[g],
                                                         mbase + (size_t)ci[g].y * P + (size_t)w * D, w2, keep,
-                                                        a.variant, lpar);
+                                                        variant, lpar);
                 }
             }
             __syncthreads();
@@ -959,9 +962,21 @@ __global__ __launch_bounds__(DMAXC <= 8 ? 1024 : 256) void k_pool_pairs(const Po
     }
 }
 
+template <bool EQ, int VK>
+static const void *pairs_rows_kernel(int P) {
+    return P == 1 ? (const void *)k_pool_pairs<EQ, 8, true, VK> : (const void *)k_pool_pairs<EQ, 8, false, VK>;
+}
 template <bool EQ>
-static const void *pairs_kernel(int dmax, int P) {
-    if (dmax <= 8) return P == 1 ? (const void *)k_pool_pairs<EQ, 8, true> : (const void *)k_pool_pairs<EQ, 8, false>;
+static const void *pairs_kernel(int dmax, int P, int variant) {
+    if (dmax <= 8) {
+        // the OMS family (OMS takes the clamp everywhere in a frame queue) and
+        // learned offsets with their magnitude map compiled in
+        if constexpr (!EQ) {
+            if (variant == VAR_OMS || variant == VAR_OMS_V2) return pairs_rows_kernel<EQ, VAR_OMS>(P);
+            if (variant == VAR_LOMS) return pairs_rows_kernel<EQ, VAR_LOMS>(P);
+        }
+        return pairs_rows_kernel<EQ, -1>(P);
+    }
     if (dmax <= 16) return (const void *)k_pool_pairs<EQ, 16, false>;
     return (const void *)k_pool_pairs<EQ, 32, false>;
 }
@@ -982,7 +997,7 @@ static const void *uniform_kernel(int ud, int G) {
 // degree classes.
 static const void *pool_kernel(const PoolArgs &a, int dmax) {
     const bool eq = a.variant == VAR_2NMS;
-    if (a.pairs) return eq ? pairs_kernel<true>(dmax, a.P) : pairs_kernel<false>(dmax, a.P);
+    if (a.pairs) return eq ? pairs_kernel<true>(dmax, a.P, a.variant) : pairs_kernel<false>(dmax, a.P, a.variant);
     if (a.uniform_deg) {
         if (!uniform_degree_supported(a.uniform_deg) || a.uniform_deg != dmax) return nullptr;
         if (eq) return uniform_kernel<true, -1>(a.uniform_deg, a.batch);
