@@ -488,7 +488,7 @@ __attribute__((amdgpu_waves_per_eu(UD == 0 && DMAXC == 10 ? 4 : UD == 0 && DMAXC
                     }
                     const uint2 pr = pool_check_any<EQ, DMAXC>(D, a.edge_lds + ci.y,
                                                                mbase + (size_t)ci.y * P + (size_t)w * D, w4, keep,
-                                                               a.variant, lpe, lpo);
+                                                               VK >= 0 ? VK : a.variant, lpe, lpo);
                     ove |= pr.x;
                     ovo |= pr.y;
                 }
@@ -1006,10 +1006,15 @@ static const void *pool_kernel(const PoolArgs &a, int dmax) {
         return uniform_kernel<false, -1>(a.uniform_deg, a.batch);
     }
     const bool p1 = a.P == 1;
+    // generic degree classes; the OMS family (OMS takes the clamp everywhere
+    // in a frame queue) with its magnitude map compiled in
+    const bool oms = a.variant == VAR_OMS || a.variant == VAR_OMS_V2;
 #define LDPC_GK(DM)                                                                                     \
     if (dmax <= DM)                                                                                     \
         return eq ? (p1 ? (const void *)k_pool<true, DM, 0, 1, -1, true> : (const void *)k_pool<true, DM, 0, 1, -1>) \
-                  : (p1 ? (const void *)k_pool<false, DM, 0, 1, -1, true> : (const void *)k_pool<false, DM, 0, 1, -1>);
+             : oms ? (p1 ? (const void *)k_pool<false, DM, 0, 1, VAR_OMS, true>                         \
+                         : (const void *)k_pool<false, DM, 0, 1, VAR_OMS>)                              \
+                   : (p1 ? (const void *)k_pool<false, DM, 0, 1, -1, true> : (const void *)k_pool<false, DM, 0, 1, -1>);
     LDPC_GK(8) LDPC_GK(10) LDPC_GK(16)
 #undef LDPC_GK
     return eq ? (const void *)k_pool<true, 32, 0, 1, -1> : (const void *)k_pool<false, 32, 0, 1, -1>;
