@@ -459,7 +459,7 @@ def run_mixed(L, args, t_start):
         L.channel_generate_range(d_in.ptr + base[k], c.n_vars, n_c[k], args.seed + k, 1.5, 3.5, c.rate, quant=1)
     L.synchronize()
     t_tune = time.perf_counter()
-    tuned = [L.pool_autotune(c, variant, max_it, ebn0_db=2.5, sample_frames=32768, quant=1) for c in codes]
+    tuned = [L.pool_autotune(c, variant, max_it, ebn0_db=2.5, sample_frames=65536, quant=1) for c in codes]
     t_tune = time.perf_counter() - t_tune
     mixed = L.MixedDecoder(codes, variant, max_it, tunings=tuned, max_frames=frames)
     step = lambda graph: mixed.decode_device(d_in.ptr, d_off.ptr, d_cid.ptr, frames, d_out.ptr, d_it.ptr,
@@ -487,7 +487,7 @@ def run_mixed(L, args, t_start):
                   "levels": c.n_levels, "frames": n_c[k], "avg_iterations": round(float(ik.mean()), 4),
                   "iterations_hist": np.bincount(ik, minlength=max_it + 1).tolist(),
                   "tuning": {kk: tuned[k][kk] for kk in ("words_per_block", "threads", "uniform", "batch", "pairs")},
-                  "tune_ms_per_32768_frames": round(tuned[k]["ms"], 4),
+                  "tune_ms_per_65536_frames": round(tuned[k]["ms"], 4),
                   "alone_ms": round(alone[k], 4),
                   "alone_info_gbps": round(n_c[k] * c.n_info / (alone[k] * 1e-3) / 1e9, 3)})
         per_code.append(r)
