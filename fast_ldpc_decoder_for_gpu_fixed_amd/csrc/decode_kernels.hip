@@ -355,8 +355,9 @@ static hipError_t launch_dclass(const DecodeArgs &a, int dmax, int threads, hipS
 template <bool EQ, int UD>
 static hipError_t launch_uniform_g(const DecodeArgs &a, int batch, int threads, hipStream_t s) {
     // the plain OMS family and learned offsets get their own straight-line
-    // instantiations; the other variants read the variant at run time
-    const int vk = (a.variant == VAR_OMS || a.variant == VAR_OMS_V2) ? VAR_OMS : a.variant == VAR_LOMS ? VAR_LOMS : -1;
+    // instantiations at every batch size; MS, NMS and 2NMS at the default
+    // batch of 2 items; the rest read the variant at run time
+    const int vk = (a.variant == VAR_OMS || a.variant == VAR_OMS_V2) ? VAR_OMS : a.variant;
 #define LDPC_UG(VKC)                                                                              \
     if (vk == VKC) {                                                                              \
         if (batch >= 4) return launch_decode_t<EQ, true, 8, UD, 4, VKC>(a, threads, s);           \
@@ -366,8 +367,11 @@ static hipError_t launch_uniform_g(const DecodeArgs &a, int batch, int threads, 
     if constexpr (!EQ) {
         LDPC_UG(VAR_OMS)
         LDPC_UG(VAR_LOMS)
+        if (batch == 2 && vk == VAR_MS) return launch_decode_t<EQ, true, 8, UD, 2, VAR_MS>(a, threads, s);
+        if (batch == 2 && vk == VAR_NMS) return launch_decode_t<EQ, true, 8, UD, 2, VAR_NMS>(a, threads, s);
+    } else {
+        if (batch == 2 && vk == VAR_2NMS) return launch_decode_t<EQ, true, 8, UD, 2, VAR_2NMS>(a, threads, s);
     }
-    LDPC_UG(-1)
 #undef LDPC_UG
     if (batch >= 4) return launch_decode_t<EQ, true, 8, UD, 4, -1>(a, threads, s);
     if (batch == 2) return launch_decode_t<EQ, true, 8, UD, 2, -1>(a, threads, s);
