@@ -11,7 +11,9 @@ info throughput.  The plain decoder is timed the same way beside it, and both
 FERs come from the device error counters on the same frames.
 
 --mode plain: the drop-in decoder alone on 2^16 frames, 20 fixed iterations
-(the plain-OMS HBM roofline line).
+(the plain-OMS HBM roofline line); config.variants times all five reference
+variants (MS, OMS, OMS_v2, NMS, 2NMS) at 20 and 10 fixed iterations on the same
+frames.
 
 --mode ira: the long IRA code (N=64800, K=32400, info degrees 8/3, staircase
 parity), 8192 frames with per-frame Eb/N0 uniform in [1.0, 1.6] dB, the
@@ -372,6 +374,29 @@ def run_plain(L, args, t_start):
     if tr:
         out["config"]["hbm_gbps"] = round(tr / (ms * 1e-3) / 1e9, 1)
         out["config"]["hbm_gbps_method"] = HBM_GBPS_METHOD
+    # the five reference variants at 10 and 20 fixed iterations on the same
+    # frames (the drop-in workload's full set), each timed the same way; the
+    # OMS_v2 decoder reads the word layout, the others frame-major frames
+    if variant == L.OMS_V2:
+        d_words, d_frames = d_in, L.DeviceBuffer(frames * N)
+        L.channel_generate(d_frames.ptr, N, frames, args.seed, 0, sigma, layout=0)
+    else:
+        d_frames, d_words = d_in, L.DeviceBuffer(frames * N)
+        L.interleave_device(d_in.ptr, d_words.ptr, N, W)
+    L.synchronize()
+    table = {}
+    for name, v in (("MS", L.MS), ("OMS", L.OMS), ("OMS_v2", L.OMS_V2), ("NMS", L.NMS), ("2NMS", L.NMS2)):
+        dv = L.make_decoder(code, W, v, early_term=0, words_per_block=args.words_per_block, threads=args.threads)
+        src = d_words.ptr if v == L.OMS_V2 else d_frames.ptr
+        row = {}
+        for it in (20, 10):
+            t = timed(L, dv.stream, min(args.steps, 20), 1, lambda: dv.decode_device(src, d_out.ptr, it))
+            c = counts(L, dv, d_out.ptr, frames, K)
+            row[f"it{it}"] = {"ms_per_decode": round(t, 4), "info_gbps": round(frames * K / (t * 1e-3) / 1e9, 3),
+                              "fer": c["fer"], "ber": c["ber"]}
+        table[name] = row
+        del dv
+    out["config"]["variants"] = table
     if not args.no_cpu_baseline and variant == L.OMS:
         n = 4 * args.cpu_baseline_words * N
         llr = d_in.download(np.empty(n, np.int8))
