@@ -531,8 +531,12 @@ def run_mixed(L, args, t_start):
                    "frames": frames, "codes": per_code, "eager_ms_per_step": round(ms_eager, 4),
                    "eager_info_gbps": round(info / (ms_eager * 1e-3) / 1e9, 3),
                    "autotune_seconds": round(t_tune, 2), "parallelism": "single GPU"},
-        "roofline": roofline(alg, ms, None),
+        "roofline": roofline(alg, ms, traffic_for(args.traffic_json, f"mixed_qc_12x24_z81_4x24_z96_f{frames}")),
     }
+    tr = out["roofline"]["traffic"]
+    if tr:  # the metric's "HBM GB/s": the step's kernels together
+        out["config"]["hbm_gbps"] = round(tr / (ms * 1e-3) / 1e9, 1)
+        out["config"]["hbm_gbps_method"] = HBM_GBPS_METHOD
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         from oracle import build as ob
