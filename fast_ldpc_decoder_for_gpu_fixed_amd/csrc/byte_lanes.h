@@ -54,10 +54,11 @@ __device__ __forceinline__ u16x2 sub_sat_u(u16x2 a, u16x2 b) {
 }
 __device__ __forceinline__ u16x2 umin(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ u16x2 umax(u16x2 a, u16x2 b) { return __builtin_elementwise_max(a, b); }
-// arithmetic shift of each half by 15: 0xFFFF where the top bit is set.
-// Written as one VALU instruction in asm so that (r ^ m) - m stays a
-// two-instruction conditional negate: left visible, the compiler rewrites
-// it into a per-half compare/select sequence (about three times longer).
+// arithmetic shift of each half by 15: 0xFFFF where the top bit is set (the
+// sign mask the check update turns into a +-1 factor, or uses in the
+// conditional negate (r ^ m) - m).  Written as one VALU instruction in asm:
+// left visible, the compiler rewrites the shift and its use into a per-half
+// compare/select sequence (about three times longer).
 __device__ __forceinline__ u16x2 sign_mask(u16x2 x) {
     uint32_t r;
     asm("v_pk_ashrrev_i16 %0, 15, %1 op_sel_hi:[0,1]" : "=v"(r) : "v"(w32(x)));
