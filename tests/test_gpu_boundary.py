@@ -10,7 +10,8 @@ compares each with the oracle bit for bit:
     after every iteration (state API);
   * k_pool: 4-frame words (generic and single-degree with 1, 2, 4 items in
     flight) and 2-frame pairs (rows layout for degrees <= 8, the generic pair
-    gather above); decisions and iteration counts per frame.
+    gather above; two pairs per workgroup and one); decisions and iteration
+    counts per frame.
 
 Semantics at stake (src/decoder_ms/cuda/simd_functions.h:89-131,
 src/decoder_oms/cuda/CUDA_OMS_SIMD.cu:166-186): |-128| = 128 never becomes a
@@ -184,6 +185,7 @@ POOL_PATHS = {
     "words_uniform_g2": dict(pairs=0, uniform=1, batch=2),
     "words_uniform_g4": dict(pairs=0, uniform=1, batch=4),
     "pairs": dict(pairs=1, words_per_block=2, threads=256),
+    "pairs_p1": dict(pairs=1, words_per_block=1, threads=256),  # the one-pair-per-workgroup instantiations
 }
 
 

@@ -4,12 +4,15 @@ bit-exact, for every variant, with slots refilled many times over (few
 workgroups, many frames), on codes that exercise each degree class, a frame
 length that is not a multiple of four, learned tables and the default grid."""
 import os
+from pathlib import Path
 
 import numpy as np
 import pytest
 
 import fast_ldpc_decoder_for_gpu_fixed_amd as L
 from oracle import oracle as O
+
+ROOT = Path(__file__).resolve().parents[1]
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
@@ -32,9 +35,9 @@ def table_for(variant, iters=20, degs=33, seed=5):
     return rng.integers(lo, hi + 1, size=(iters, degs)).astype(np.int8)
 
 
-def check_pool(code, variant, llr, max_it, workgroups=0, stop_rule=L.STOP_SYNDROME, **tuning):
+def check_pool(code, variant, llr, max_it, workgroups=0, stop_rule=L.STOP_SYNDROME, table=None, **tuning):
     deg, pos = code.export()
-    table = table_for(variant)
+    table = table_for(variant) if table is None else table
     pool = L.PoolDecoder(code, variant, max_it, table=table, stop_rule=stop_rule, **tuning)
     hard, its = pool.decode(llr, workgroups=workgroups)
     ref_h, ref_i = O.decode_frames(deg, pos, code.n_vars, L.OMS_V2 if variant == L.OMS else variant, llr, max_it,
@@ -235,6 +238,18 @@ def test_pool_ira_n64800_auto_pairs(gpu_available):
     assert cfg["pairs"] == 1 and cfg["words_per_block"] == 1
     its = check_pool(c, L.OMS_V2, llr, 50)
     assert its.max() < 50
+
+
+def test_pool_ira_n64800_learned_table(gpu_available):
+    """The IRA code with the committed learned table (the bench's learned leg:
+    one pair per workgroup, the learned-offset kernel) against the oracle."""
+    from fast_ldpc_decoder_for_gpu_fixed_amd.learned import load_table
+    variant, table = load_table(str(ROOT / "fast_ldpc_decoder_for_gpu_fixed_amd" / "tables" / "loms_ira_n64800.json"))
+    c = L.Code.ira(32400, 12960, 8, 3, 1).reorder_by_colour()
+    llr = ira_llr(c, 24, 1.1, 6)
+    pool = L.PoolDecoder(c, variant, 50, table=table)
+    assert pool.config()["pairs"] == 1 and pool.config()["words_per_block"] == 1
+    check_pool(c, variant, llr, 50, table=table)
 
 
 # ---------------------------------------------------------------------------
