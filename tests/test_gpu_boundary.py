@@ -181,6 +181,7 @@ def test_decoder_state_at_boundary_values(path, gpu_available):
 
 POOL_PATHS = {
     "words_generic": dict(pairs=0, uniform=0),
+    "words_generic_p1": dict(pairs=0, uniform=0, words_per_block=1),  # one word per workgroup
     "words_uniform_g1": dict(pairs=0, uniform=1, batch=1),
     "words_uniform_g2": dict(pairs=0, uniform=1, batch=2),
     "words_uniform_g4": dict(pairs=0, uniform=1, batch=4),
