@@ -418,9 +418,13 @@ def run_ira(L, args, t_start):
     probe = L.make_decoder(code, 4, L.OMS_V2)
     legs = {}
     d_hard, d_it = L.DeviceBuffer(frames * N), L.DeviceBuffer(4 * frames)
-    for name, v, tab in (("learned_pool", variant, table), ("oms_pool", L.OMS_V2, None)):
+    # the frame queues take the noisiest frames first (ldpc_pool_set_order); the learned decoder is also timed
+    # in input order, the queue order of earlier measurements
+    lrf, inp = L.ORDER_LEAST_RELIABLE_FIRST, L.ORDER_INPUT
+    for name, v, tab, order in (("learned_pool", variant, table, lrf), ("oms_pool", L.OMS_V2, None, lrf),
+                                ("learned_pool_input_order", variant, table, inp)):
         pool = L.PoolDecoder(code, v, max_it, table=tab, words_per_block=args.words_per_block,
-                             threads=args.threads or 1024)
+                             threads=args.threads or 1024, order=order)
         ms = timed(L, pool.stream, args.steps, args.warmup,
                    lambda: pool.decode_device(d_llr.ptr, frames, d_hard.ptr, d_it.ptr))
         its = d_it.download(np.empty(frames, np.int32)).astype(np.int64)
@@ -428,10 +432,12 @@ def run_ira(L, args, t_start):
         alg = int(its.sum()) * (2 * E + 2 * N) + frames * 2 * N
         r.update({"info_gbps": round(frames * K / (ms * 1e-3) / 1e9, 3), "ms_per_step": round(ms, 4),
                   "avg_iterations": round(float(its.mean()), 4),
+                  "order": "least reliable first" if order == lrf else "input",
                   "roofline": roofline(alg, ms, traffic_for(args.traffic_json, f"ira_pool_n64800_f{frames}")
                                        if name == "learned_pool" else None)})
         r.update(pool.config())
         legs[name] = r
+        del pool
     dec = L.make_decoder(code, frames // 4, L.OMS_V2, early_term=0)
     d_words = L.DeviceBuffer(frames * N)  # OMS_v2 takes the device word layout (the reference's xMS convention)
     L.interleave_device(d_llr.ptr, d_words.ptr, N, frames // 4)

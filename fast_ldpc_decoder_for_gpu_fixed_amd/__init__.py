@@ -23,6 +23,7 @@ from ._lib import LdpcError, LearnConfig, SimStats, Tuning, check, load, ptr
 
 MS, OMS, OMS_V2, NMS, NMS2, LOMS, LNMS = range(7)
 STOP_SYNDROME, STOP_V2C_PARITY, STOP_NONE = 0, 1, 2  # frame-queue decoder stop tests
+ORDER_INPUT, ORDER_LEAST_RELIABLE_FIRST = 0, 1  # frame-queue decoder queue orders
 VARIANT_NAMES = {MS: "MS", OMS: "OMS", OMS_V2: "OMS_v2", NMS: "NMS", NMS2: "2NMS",
                  LOMS: "learned-OMS", LNMS: "learned-NMS"}
 
@@ -377,9 +378,11 @@ class PoolDecoder:
 
     def __init__(self, code: Code, variant: int, max_iterations: int, words_per_block: int = 0,
                  threads: int = 0, table=None, uniform: int = -1, batch: int = 0, stop_rule: int = 0,
-                 pairs: int = -1):
+                 pairs: int = -1, order: int = ORDER_INPUT):
         """pairs: -1 auto (pairs of frames only when a 4-frame word block does
-        not fit LDS), 0 words of 4 frames, 1 pairs of frames."""
+        not fit LDS), 0 words of 4 frames, 1 pairs of frames.  order: queue
+        order of a decode call's frames (ORDER_LEAST_RELIABLE_FIRST starts the
+        noisiest frames first; results are the same in either order)."""
         tu = Tuning(words_per_block, threads, -1, {-1: -1, 0: 1, 1: 2}[pairs], uniform, batch)
         h = load().ldpc_pool_create(code.handle, variant, max_iterations, stop_rule, C.byref(tu))
         if not h:
@@ -388,6 +391,8 @@ class PoolDecoder:
         self.code, self.variant, self.max_iterations, self.stop_rule = code, variant, max_iterations, stop_rule
         if table is not None:
             self.set_table(table)
+        if order != ORDER_INPUT:
+            self.set_order(order)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -399,6 +404,9 @@ class PoolDecoder:
         t = np.ascontiguousarray(table, dtype=np.int8)
         assert t.ndim == 2
         check(load().ldpc_pool_set_table(self._h, ptr(t, C.c_int8), t.shape[0], t.shape[1]), "pool_set_table")
+
+    def set_order(self, order: int) -> None:
+        check(load().ldpc_pool_set_order(self._h, order), "pool_set_order")
 
     def decode_device(self, d_in: int, frames: int, d_out: int, d_iters: int = 0, workgroups: int = 0,
                       stream: int = 0, graph: bool = False) -> None:

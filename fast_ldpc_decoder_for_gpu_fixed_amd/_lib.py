@@ -80,6 +80,7 @@ SIGNATURES = {
     "ldpc_trainer_fit": (C.c_int, [C.c_void_p, C.POINTER(LearnConfig), _f_p, _f_p]),
     "ldpc_pool_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning)]),
     "ldpc_pool_set_table": (C.c_int, [C.c_void_p, _i8_p, C.c_int, C.c_int]),
+    "ldpc_pool_set_order": (C.c_int, [C.c_void_p, C.c_int]),
     "ldpc_pool_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ldpc_pool_decode_graph": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "ldpc_pool_simulate": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64,

@@ -109,6 +109,11 @@ hipError_t launch_pool(const PoolArgs &a, int dmax, int grid, int threads, hipSt
 // (-1: a pair-mode rows kernel with static LDS, which its addressing forbids)
 int pool_prepare(const PoolArgs &a, int dmax, int threads);
 hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t s);
+// queue order least reliable first: scratch of order_bytes(frames) bytes
+// (4-byte aligned); *list_out receives the queue-position -> frame-id list in it
+size_t order_bytes(unsigned long long frames);
+hipError_t launch_order(const int8_t *in, int N, unsigned long long frames, void *scratch, hipStream_t s,
+                        const unsigned **list_out);
 
 // float shadow decoder (learned tables)
 struct ShadowArgs {

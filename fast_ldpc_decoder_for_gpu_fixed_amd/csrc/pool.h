@@ -26,12 +26,15 @@ struct ldpc_pool {
     uint32_t *d_rows = nullptr;  // pair mode with degrees <= 8: LDS addresses as rows of 8 per check
     int *d_level_deg = nullptr;  // degree shared by every check of a level, or 0
     int n_cu = 1;   // compute units of the device
+    int order = 0;  // LDPC_ORDER_*: queue order of the frames of a launch
+    void *d_order = nullptr;  // its scratch (ldpc::order_bytes of the largest batch so far)
+    size_t order_cap = 0;
     // captured decode step (ldpc_pool_decode_graph) and what it was captured with
     hipStream_t capture = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     struct {
-        const void *in, *out, *iters, *table;
+        const void *in, *out, *iters, *table, *order;
         unsigned long long frames;
         int grid, max_iters;
     } key = {};
@@ -50,6 +53,11 @@ struct PoolIO {
     const unsigned long long *frame_off = nullptr;
 };
 PoolArgs pool_args(const ldpc_pool *p);
-// zero the pool's queue head and launch its kernel on s (capturable: kernels only)
+// zero the pool's queue head and launch its kernel on s (capturable: kernels
+// only).  With the pool's order set and no frame list in io, the order
+// kernels run first and the launch takes its frames through their list: the
+// order scratch must already hold order_bytes(io.total) (pool_reserve_order).
 hipError_t pool_enqueue(ldpc_pool *p, const PoolIO &io, int grid, hipStream_t s);
+// grow the pool's order scratch to `frames` frames (allocates: not under capture)
+hipError_t pool_reserve_order(ldpc_pool *p, unsigned long long frames);
 }  // namespace ldpc

@@ -47,6 +47,7 @@ void pool_free(ldpc_pool *p) {
     (void)hipFree(p->d_edge_off16);
     (void)hipFree(p->d_rows);
     (void)hipFree(p->d_level_deg);
+    (void)hipFree(p->d_order);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -97,7 +98,24 @@ hipError_t pool_enqueue(ldpc_pool *p, const PoolIO &io, int grid, hipStream_t s)
     a.total_ptr = io.total_ptr;
     a.frame_list = io.frame_list;
     a.frame_off = io.frame_off;
+    if (p->order == LDPC_ORDER_LEAST_RELIABLE_FIRST && !io.frame_list && !io.frame_off && !io.total_ptr) {
+        if (p->order_cap < order_bytes(io.total)) return hipErrorInvalidValue;
+        e = launch_order(io.in, p->code.N, io.total, p->d_order, s, &a.frame_list);
+        if (e != hipSuccess) return e;
+    }
     return launch_pool(a, p->code.max_deg, grid, p->threads, s);
+}
+
+hipError_t pool_reserve_order(ldpc_pool *p, unsigned long long frames) {
+    const size_t need = order_bytes(frames);
+    if (p->order_cap >= need) return hipSuccess;
+    drop_graph(p);  // a captured step holds the old scratch's address
+    (void)hipFree(p->d_order);
+    p->d_order = nullptr;
+    p->order_cap = 0;
+    const hipError_t e = hipMalloc(&p->d_order, need);
+    if (e == hipSuccess) p->order_cap = need;
+    return e;
 }
 }  // namespace ldpc
 
@@ -294,6 +312,16 @@ int ldpc_pool_set_table(ldpc_pool *p, const int8_t *table, int n_iters, int n_de
     return LDPC_OK;
 }
 
+int ldpc_pool_set_order(ldpc_pool *p, int order) {
+    if (!p || (order != LDPC_ORDER_INPUT && order != LDPC_ORDER_LEAST_RELIABLE_FIRST)) {
+        ldpc::set_last_error("ldpc_pool_set_order: bad arguments");
+        return LDPC_ERR_ARG;
+    }
+    if (order != p->order) drop_graph(p);  // a captured step holds the old order's kernels
+    p->order = order;
+    return LDPC_OK;
+}
+
 static int pool_decode_args(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t *d_out, int workgroups,
                             const char *who, int *grid) {
     if (!p || !d_in || !d_out || frames == 0) {
@@ -316,6 +344,7 @@ int ldpc_pool_decode(ldpc_pool *p, const int8_t *d_in, uint64_t frames, uint8_t 
     const int rc = pool_decode_args(p, d_in, frames, d_out, workgroups, "ldpc_pool_decode", &grid);
     if (rc) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    if (p->order) PHIP(ldpc::pool_reserve_order(p, frames));
     ldpc::PoolIO io;
     io.in = d_in;
     io.out = d_out;
@@ -331,9 +360,11 @@ int ldpc_pool_decode_graph(ldpc_pool *p, const int8_t *d_in, uint64_t frames, ui
     int rc = pool_decode_args(p, d_in, frames, d_out, workgroups, "ldpc_pool_decode_graph", &grid);
     if (rc) return rc;
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    if (p->order) PHIP(ldpc::pool_reserve_order(p, frames));
+    const void *order = p->order ? p->d_order : nullptr;
     const bool same = p->exec && p->key.in == d_in && p->key.out == d_out && p->key.iters == d_iters &&
                       p->key.table == p->d_table && p->key.frames == frames && p->key.grid == grid &&
-                      p->key.max_iters == p->max_iters;
+                      p->key.max_iters == p->max_iters && p->key.order == order;
     if (!same) {
         // capture once: the queue-head reset kernel and the decode kernel
         // (kernels only, no memset node), on the pool's capture stream
@@ -368,6 +399,7 @@ int ldpc_pool_decode_graph(ldpc_pool *p, const int8_t *d_in, uint64_t frames, ui
         p->key.frames = frames;
         p->key.grid = grid;
         p->key.max_iters = p->max_iters;
+        p->key.order = order;
     }
     PHIP(hipGraphLaunch(p->exec, s));
     return LDPC_OK;

@@ -1056,4 +1056,136 @@ hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Queue order: least reliable frames first.  A frame-queue launch ends when
+// its last frames do; taken in input order, the last frames of a batch can
+// need several times the mean iteration count while the other workgroups have
+// drained.  The sum of squares of a frame's first ORDER_SAMPLE LLRs grows with
+// its noise power, so frames are queued by it, largest first: a counting sort
+// over ORDER_BUCKETS buckets spread over the batch's own key range (frames of
+// one bucket in any order).  The frames that run longest start first and a
+// batch ends on short ones; each frame decodes exactly as in any order.
+// ---------------------------------------------------------------------------
+constexpr int ORDER_SAMPLE = 4096, ORDER_BUCKETS = 1024;
+
+// ORDER_FPW frames per wave, 4 waves per workgroup: every frame's key, and
+// the batch's key range (one atomic pair per workgroup: a pair per frame on
+// two addresses serialises in L2, about 24 ns a frame)
+constexpr int ORDER_FPW = 4;
+__global__ __launch_bounds__(256) void k_order_keys(const int8_t *__restrict__ in, int N, unsigned long long frames,
+                                                    unsigned *__restrict__ key, unsigned *__restrict__ range) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = min(N, ORDER_SAMPLE);
+    const bool dwords = (N & 3) == 0 && (reinterpret_cast<uintptr_t>(in) & 3) == 0;
+    unsigned lo = 0xFFFFFFFFu, hi = 0u;
+    for (int j = 0; j < ORDER_FPW; j++) {
+        const unsigned long long f = ((unsigned long long)blockIdx.x * 4 + wave) * ORDER_FPW + j;
+        if (f >= frames) break;  // wave-uniform
+        const int8_t *x = in + f * (unsigned long long)N;
+        unsigned acc = 0;
+        if (dwords) {
+            const uint32_t *x4 = reinterpret_cast<const uint32_t *>(x);
+#pragma unroll 8
+            for (int q = lane; q < (n >> 2); q += 64) {
+                const uint32_t w = x4[q];
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int y = (int8_t)(w >> (8 * b));
+                    acc += (unsigned)(y * y);
+                }
+            }
+        } else {
+            for (int v = lane; v < n; v += 64) {
+                const int y = x[v];
+                acc += (unsigned)(y * y);
+            }
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d);  // <= 4096 * 16384 = 2^26
+        if (lane == 0) key[f] = acc;
+        lo = min(lo, acc);
+        hi = max(hi, acc);
+    }
+    __shared__ unsigned part[2][4];
+    if (lane == 0) {
+        part[0][wave] = lo;
+        part[1][wave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        lo = min(min(part[0][0], part[0][1]), min(part[0][2], part[0][3]));
+        hi = max(max(part[1][0], part[1][1]), max(part[1][2], part[1][3]));
+        if (lo <= hi) {  // the workgroup had a frame
+            atomicMin(&range[0], lo);
+            atomicMax(&range[1], hi);
+        }
+    }
+}
+
+// bucket of a key: 0 for the largest key of the batch
+__device__ __forceinline__ unsigned order_bucket(unsigned k, const unsigned *range) {
+    const unsigned long long span = (unsigned long long)(range[1] - range[0]) + 1ull;
+    return (unsigned)(ORDER_BUCKETS - 1) - (unsigned)((unsigned long long)(k - range[0]) * ORDER_BUCKETS / span);
+}
+
+__global__ void k_order_count(const unsigned *__restrict__ key, unsigned long long frames,
+                              const unsigned *__restrict__ range, unsigned *__restrict__ counts) {
+    const unsigned long long f = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < frames) atomicAdd(&counts[order_bucket(key[f], range)], 1u);
+}
+
+// exclusive prefix sum of the bucket counts into the bucket starts (one workgroup)
+__global__ __launch_bounds__(ORDER_BUCKETS) void k_order_scan(const unsigned *__restrict__ counts,
+                                                             unsigned *__restrict__ starts) {
+    __shared__ unsigned c[ORDER_BUCKETS];
+    const int t = threadIdx.x;
+    c[t] = counts[t];
+    __syncthreads();
+    for (int d = 1; d < ORDER_BUCKETS; d <<= 1) {
+        const unsigned v = t >= d ? c[t - d] : 0u;
+        __syncthreads();
+        c[t] += v;
+        __syncthreads();
+    }
+    starts[t] = c[t] - counts[t];
+}
+
+// queue position of every frame: its bucket's start plus its rank there
+__global__ void k_order_scatter(const unsigned *__restrict__ key, unsigned long long frames,
+                                const unsigned *__restrict__ range, unsigned *__restrict__ starts,
+                                unsigned *__restrict__ list) {
+    const unsigned long long f = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < frames) list[atomicAdd(&starts[order_bucket(key[f], range)], 1u)] = (unsigned)f;
+}
+
+// the range starts as [UINT_MAX, 0]
+__global__ void k_order_init(unsigned *counts, unsigned *range) {
+    for (int i = threadIdx.x; i < ORDER_BUCKETS; i += blockDim.x) counts[i] = 0u;
+    if (threadIdx.x == 0) {
+        range[0] = 0xFFFFFFFFu;
+        range[1] = 0u;
+    }
+}
+
+size_t order_bytes(unsigned long long frames) { return ((size_t)frames * 2 + 2 * ORDER_BUCKETS + 2) * 4; }
+
+hipError_t launch_order(const int8_t *in, int N, unsigned long long frames, void *scratch, hipStream_t s,
+                        const unsigned **list_out) {
+    if (frames == 0 || frames > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    unsigned *list = static_cast<unsigned *>(scratch);
+    unsigned *key = list + frames;
+    unsigned *counts = key + frames;
+    unsigned *starts = counts + ORDER_BUCKETS;
+    unsigned *range = starts + ORDER_BUCKETS;
+    const dim3 g((unsigned)((frames + 255) / 256));
+    hipLaunchKernelGGL(k_order_init, dim3(1), dim3(256), 0, s, counts, range);
+    hipLaunchKernelGGL(k_order_keys, dim3((unsigned)((frames + 4 * ORDER_FPW - 1) / (4 * ORDER_FPW))), dim3(256), 0, s,
+                       in, N, frames, key, range);
+    hipLaunchKernelGGL(k_order_count, g, dim3(256), 0, s, key, frames, range, counts);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(ORDER_BUCKETS), 0, s, counts, starts);
+    hipLaunchKernelGGL(k_order_scatter, g, dim3(256), 0, s, key, frames, range, starts, list);
+    *list_out = list;
+    return hipGetLastError();
+}
+
 }  // namespace ldpc

@@ -176,6 +176,8 @@ public:
     void set_table(const int8_t *table, int n_iters, int n_degs) {
         ldpc::check(ldpc_pool_set_table(p_, table, n_iters, n_degs), "ldpc_pool_set_table");
     }
+    // LDPC_ORDER_INPUT or LDPC_ORDER_LEAST_RELIABLE_FIRST (same results, other queue order)
+    void set_order(int order) { ldpc::check(ldpc_pool_set_order(p_, order), "ldpc_pool_set_order"); }
     // frame-major device buffers; d_iters may be null
     void decode_device(const int8_t *d_in, uint64_t frames, uint8_t *d_out, int *d_iters = nullptr,
                        void *stream = nullptr, int workgroups = 0) {

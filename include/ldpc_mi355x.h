@@ -274,6 +274,20 @@ typedef struct ldpc_pool ldpc_pool;
 ldpc_pool *ldpc_pool_create(const ldpc_code *code, int variant, int max_iterations, int stop_rule,
                             const ldpc_tuning *tuning);
 int ldpc_pool_set_table(ldpc_pool *pool, const int8_t *table, int n_iters, int n_degs);
+/* Queue order of the frames of one ldpc_pool_decode / ldpc_pool_decode_graph
+ * call (no counterpart in the reference, whose words all run the same fixed
+ * number of iterations).  LDPC_ORDER_INPUT: frame i is queued i-th.
+ * LDPC_ORDER_LEAST_RELIABLE_FIRST: frames are queued by the sum of squares of
+ * their first 4096 LLRs, largest first (noisiest first, in 1024 buckets over
+ * the call's own range of that sum), so the frames that need many iterations
+ * start early and a call ends on short ones.  Every frame's decisions and
+ * iteration count are the same in either order.  The order needs
+ * 8 bytes per frame of device scratch, allocated by the decode call when the
+ * call has more frames than any before (so the first call at a new largest
+ * size is not capturable by the caller; ldpc_pool_decode_graph allocates
+ * before its own capture); frames above 2^32 - 1 are LDPC_ERR_HIP. */
+enum { LDPC_ORDER_INPUT = 0, LDPC_ORDER_LEAST_RELIABLE_FIRST = 1 };
+int ldpc_pool_set_order(ldpc_pool *pool, int order);
 /* Decode `frames` frame-major LLRs at d_in into d_out (decisions) and
  * d_iters (iterations per frame, may be NULL).  workgroups = 0: as many as
  * stay resident on the GPU (capped by the work).  Asynchronous on `stream`

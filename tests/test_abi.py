@@ -79,6 +79,7 @@ int main() {
     delete d;
     {
         CFrameQueueDecoder q(code, LDPC_OMS, 20);
+        q.set_order(LDPC_ORDER_LEAST_RELIABLE_FIRST);
         ldpc_sim_stats s = q.simulate(2.5, 4096);
         if (s.frames != 4096) return 2;
         const ldpc_code* codes[1] = {code};
@@ -205,3 +206,8 @@ def test_cpp_dropin_decodes_every_variant(tmp_path, gpu_available):
         orc.iterate(iters, threads=8)
         got = np.frombuffer((tmp_path / f"out_{name}.bin").read_bytes(), np.uint8)
         np.testing.assert_array_equal(got, orc.hard(), err_msg=name)
+
+
+def test_pool_set_order_rejects_null_pool():
+    assert L.load().ldpc_pool_set_order(None, 0) == -1  # LDPC_ERR_ARG
+    assert "ldpc_pool_set_order" in _lib.last_error()

@@ -74,8 +74,13 @@ def test_ira_and_mixed_modes_small(gpu_available):
     d = bench("--mode", "ira", "--frames", "1024", "--steps", "2", "--warmup", "1", "--no-cpu-baseline")
     check_line(d, 2)
     legs = d["config"]["legs"]
-    assert set(legs) == {"learned_pool", "oms_pool", "oms50_fixed"}
+    assert set(legs) == {"learned_pool", "oms_pool", "learned_pool_input_order", "oms50_fixed"}
     assert legs["oms50_fixed"]["avg_iterations"] == 50
+    # the queue order changes no frame's result: same counts in either order
+    a, b = legs["learned_pool"], legs["learned_pool_input_order"]
+    assert a["order"] == "least reliable first" and b["order"] == "input"
+    assert (a["avg_iterations"], a["bit_errors"], a["frame_errors"]) == \
+        (b["avg_iterations"], b["bit_errors"], b["frame_errors"])
     d = bench("--mode", "mixed", "--frames", "8192", "--steps", "2", "--warmup", "1", "--no-cpu-baseline")
     check_line(d, 2)
     codes = d["config"]["codes"]

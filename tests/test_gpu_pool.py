@@ -319,3 +319,63 @@ def test_pool_graph_replays_match_eager_and_oracle(gpu_available):
     h, i = run(graph=True)
     np.testing.assert_array_equal(i, r2_i)
     np.testing.assert_array_equal(h, r2_h)
+
+
+# ---------------------------------------------------------------------------
+# queue order: least reliable frames first (ldpc_pool_set_order)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("fmt", ["words", "pairs"])
+def test_pool_order_least_reliable_first_matches_oracle(fmt, gpu_available):
+    """Frames queued noisiest first (a per-frame Eb/N0 spread, so the order
+    differs from the input's): every frame's decisions and iteration count
+    equal the oracle's and the input-order decode's, with few workgroups (many
+    refills) and with the default grid."""
+    c = code("qc_12x24_z27_d9")
+    F = 1500
+    llr = mixed_llr(c.n_vars, F, seed=41, lo=0.8, hi=3.2)
+    tuning = dict(pairs=1 if fmt == "pairs" else 0, words_per_block=2)
+    for wg in (3, 0):
+        ref_i = check_pool(c, L.OMS_V2, llr, 30, workgroups=wg, **tuning)
+        pool = L.PoolDecoder(c, L.OMS_V2, 30, order=L.ORDER_LEAST_RELIABLE_FIRST, **tuning)
+        hard, its = pool.decode(llr, workgroups=wg)
+        np.testing.assert_array_equal(its, ref_i, err_msg=f"ordered iterations, workgroups={wg}")
+        deg, pos = c.export()
+        ref_h, _ = O.decode_frames(deg, pos, c.n_vars, L.OMS_V2, llr, 30, threads=THREADS)
+        np.testing.assert_array_equal(hard, ref_h, err_msg=f"ordered decisions, workgroups={wg}")
+    assert len(set(ref_i.tolist())) > 3  # iteration counts spread: the order matters to the queue
+
+
+def test_pool_order_ira_learned_and_graph(gpu_available):
+    """The long IRA code in pair mode with its learned table, ordered, eager
+    and as a replayed graph; a larger call regrows the order scratch (the
+    graph is recaptured), a smaller one reuses it; switching the order back
+    recaptures too.  Every result equals the oracle's."""
+    from fast_ldpc_decoder_for_gpu_fixed_amd.learned import load_table
+    variant, table = load_table(str(ROOT / "fast_ldpc_decoder_for_gpu_fixed_amd" / "tables" / "loms_ira_n64800.json"))
+    c = L.Code.ira(32400, 12960, 8, 3, 1).reorder_by_colour()
+    deg, pos = c.export()
+    F = 40
+    llr = ira_llr(c, F, 1.2, 8)
+    ref_h, ref_i = O.decode_frames(deg, pos, c.n_vars, variant, llr, 50, table=table, threads=THREADS)
+    pool = L.PoolDecoder(c, variant, 50, table=table, order=L.ORDER_LEAST_RELIABLE_FIRST)
+    d_in, d_out, d_it = L.DeviceBuffer(llr.nbytes), L.DeviceBuffer(llr.nbytes), L.DeviceBuffer(4 * F)
+    d_in.upload(llr)
+
+    def run(frames, graph):
+        d_out.upload(np.full(llr.nbytes, 0xAA, np.uint8))
+        d_it.upload(np.full(F, -1, np.int32))
+        pool.decode_device(d_in.ptr, frames, d_out.ptr, d_it.ptr, graph=graph)
+        L.synchronize()
+        h, i = d_out.download(np.empty((F, c.n_vars), np.uint8)), d_it.download(np.empty(F, np.int32))
+        np.testing.assert_array_equal(i[:frames], ref_i[:frames], err_msg=f"frames={frames} graph={graph}")
+        np.testing.assert_array_equal(h[:frames], ref_h[:frames], err_msg=f"frames={frames} graph={graph}")
+        assert (i[frames:] == -1).all()
+
+    for frames, graph in ((16, False), (16, True), (16, True), (F, True), (F, False), (24, True), (F, True)):
+        run(frames, graph)
+    pool.set_order(L.ORDER_INPUT)
+    run(F, True)
+    pool.set_order(L.ORDER_LEAST_RELIABLE_FIRST)
+    run(F, True)
+    with pytest.raises(L.LdpcError):
+        pool.set_order(2)
