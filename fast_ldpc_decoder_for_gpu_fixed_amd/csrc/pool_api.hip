@@ -107,6 +107,7 @@ hipError_t pool_enqueue(ldpc_pool *p, const PoolIO &io, int grid, hipStream_t s)
 }
 
 hipError_t pool_reserve_order(ldpc_pool *p, unsigned long long frames) {
+    if (frames > 0xFFFFFFFFull) return hipErrorInvalidValue;  // queue positions are 32-bit frame ids
     const size_t need = order_bytes(frames);
     if (p->order_cap >= need) return hipSuccess;
     drop_graph(p);  // a captured step holds the old scratch's address
