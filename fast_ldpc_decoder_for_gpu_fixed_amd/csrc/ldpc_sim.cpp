@@ -14,6 +14,7 @@
 //   -quant 0|1        LLR quantizer: 0 = sat127(round(16y/sigma^2)), 1 = the
 //                     reference's trunc(8y) clamped to +-31 (src/awgn_channel/CChanel_AWGN_SIMD.cu:17-28)
 //   -pool             the frame-queue decoder (per-frame syndrome stop, -iter as the limit)
+//   -order            with -pool: queue each batch's frames least reliable first
 //   -maxframes F      frames per point at most (0: no cap)
 //   -seed S
 // Per point: the device channel draws 4W frames per batch (a new Philox
@@ -47,6 +48,7 @@ struct Options {
     int n_vars = 0;
     int quant = 0;
     bool pool = false;
+    bool order = false;
     unsigned long long max_frames = 0;
     unsigned long long seed = 1;
 };
@@ -54,7 +56,7 @@ struct Options {
 void usage() {
     std::printf("usage: ldpc_sim [-min dB] [-max dB] [-pas dB] [-iter n] [-fer n] [-timer s] [-qef ber]\n"
                 "                [-n words] [-Eb/N0|-Es/N0] [-bpsk|-qpsk] [-fMS|-MS|-OMS|-NMS|-2NMS|-xMS]\n"
-                "                [-code 3_6|PATH] [-N n] [-quant 0|1] [-pool] [-maxframes F] [-seed S] [-info]\n");
+                "                [-code 3_6|PATH] [-N n] [-quant 0|1] [-pool [-order]] [-maxframes F] [-seed S] [-info]\n");
 }
 
 int fail(const char *what) {
@@ -92,6 +94,7 @@ int main(int argc, char **argv) {
         else if (!std::strcmp(a, "-N") && has) o.n_vars = std::atoi(argv[++p]);
         else if (!std::strcmp(a, "-quant") && has) o.quant = std::atoi(argv[++p]);
         else if (!std::strcmp(a, "-pool")) o.pool = true;
+        else if (!std::strcmp(a, "-order")) o.order = true;
         else if (!std::strcmp(a, "-maxframes") && has) o.max_frames = std::strtoull(argv[++p], nullptr, 10);
         else if (!std::strcmp(a, "-seed") && has) o.seed = std::strtoull(argv[++p], nullptr, 10);
         else if (!std::strcmp(a, "-info")) {
@@ -131,6 +134,7 @@ int main(int argc, char **argv) {
     if (o.pool) {
         pool = ldpc_pool_create(code, o.variant, o.iters, LDPC_STOP_SYNDROME, nullptr);
         if (!pool) return fail("pool");
+        if (o.order && ldpc_pool_set_order(pool, LDPC_ORDER_LEAST_RELIABLE_FIRST) != LDPC_OK) return fail("order");
     } else {
         dec = ldpc_decoder_create(code, o.words, o.variant, nullptr);
         if (!dec || ldpc_decoder_initialize(dec) != LDPC_OK) return fail("decoder");

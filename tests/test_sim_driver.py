@@ -49,11 +49,12 @@ def test_no_device_is_an_error_here():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("pool", [False, True], ids=["drop_in", "frame_queue"])
+@pytest.mark.parametrize("pool", [False, True, "order"], ids=["drop_in", "frame_queue", "frame_queue_ordered"])
 def test_one_batch_point_equals_library_simulation(pool, gpu_available):
+    """The queue order (-order) changes no frame's result: same counts as the library's input-order run."""
     W = 256
     args = ["-OMS", "-min", "1.0", "-max", "1.0", "-fer", "1", "-n", str(W), "-iter", "12", "-seed", "7"]
-    r = run(*(args + (["-pool"] if pool else [])))
+    r = run(*(args + (["-pool"] if pool else []) + (["-order"] if pool == "order" else [])))
     assert r.returncode == 0, r.stdout + r.stderr
     (snr, fer, frames, fe, be, _), = points(r.stdout)
     assert frames == 4 * W and fe >= 1  # 1.0 dB: the first batch already has frame errors
